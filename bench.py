@@ -1,0 +1,151 @@
+"""Benchmark: closed-shell (T) time-to-solution on one MI355X.
+
+One step = one full solve of E(T) = E[T] + E_ST (and v1) from the reference-layout input
+arrays already resident in HBM: packing of the operand slabs, the persistent FP64 MFMA
+launch over all i>=j>=k tuples, and the deterministic reductions.  Default workload:
+no=24 occupied, nu=216 virtual (2024 distinct + 552 degenerate tuples), the size the
+time-to-solution and matrix-peak targets are quoted at.  Inputs are the seeded synthetic
+closed-shell generator (libcc_amd.synth, seed 1004).
+
+Prints one JSON line.  `roofline` is for the persistent kernel (HIP events on the
+library's stream): achieved = algorithmic GEMM FLOPs / kernel time, against MI355X's
+published dense FP64 matrix peak.  `cpu_baseline` times the reference's own compiled
+per-tuple routines (oracle/_ref, threaded MKL) on a bounded sample of the same tuples.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "(T) time-to-solution & matrix-peak %; split-prec vs FP64 at |ΔE(T)|<1 µEh"
+PEAK_FP64_TFLOPS = 78.6
+CONFIGS = {"c1": (4, 36, 1001), "c2": (4, 87, 1002), "c3": (8, 120, 1003), "c4": (24, 216, 1004),
+           "c5": (30, 400, 1005)}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(inp_dev: dict, no: int, nu: int, budget_s: float, threads: int) -> dict | None:
+    """Reference per-tuple routines on a bounded sample of distinct tuples, extrapolated
+    to the whole tuple set by FLOP weight (a degenerate tuple is half a distinct one)."""
+    import numpy as np
+
+    from oracle import ref
+
+    if not ref.available():
+        log("cpu_baseline: oracle/_ref/ref_triples not built; skipped")
+        return None
+    host = {k: inp_dev[k].cpu().numpy() for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    host["no"], host["nu"] = no, nu
+    vv = inp_dev["vvvo"].cpu().numpy().reshape(nu, nu, nu, no, order="F")
+    p, q = np.tril_indices(nu)
+    packed = vv[p, q, :, :].reshape(len(p), nu * no, order="F")
+    del vv
+    host["vvvo"] = None
+    n_dist = no * (no - 1) * (no - 2) // 6
+    n_deg = no * (no - 1)
+    # calibrate on 2 tuples, then size the sample to the budget
+    cal = ref.triples(host, 0, min(2, n_dist), degenerate=False, threads=threads, packed=packed)
+    per = cal["seconds"] / max(1, min(2, n_dist))
+    sample = int(max(2, min(n_dist, budget_s / max(per, 1e-6))))
+    r = ref.triples(host, 0, sample, degenerate=False, threads=threads, packed=packed)
+    per = r["seconds"] / sample
+    full = per * (n_dist + 0.5 * n_deg)
+    return {"value": full, "unit": "s", "cores": threads, "kind": "reference",
+            "sample": f"{sample} of {n_dist} distinct tuples timed ({r['seconds']:.1f} s in the reference's "
+                      f"ijk_tuple + t1wt3_ijk with MKL on {threads} threads), extrapolated by FLOP weight to "
+                      f"{n_dist} distinct + {n_deg} degenerate tuples",
+            "tuples_per_s": 1.0 / per}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this path's tuples would shard over GPUs with one final "
+              f"reduction (see DESIGN.md), but no multi-GPU run is part of this build (--gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from libcc_amd import synth
+    from libcc_amd.triples import TriplesContext, algorithmic_flops
+
+    no, nu, seed = CONFIGS[args.config]
+    log(f"generating no={no} nu={nu} seed={seed} on cuda:0")
+    inp = synth.make_inputs_torch(no, nu, seed, "cuda:0")
+    torch.cuda.synchronize()
+    ctx = TriplesContext(no, nu)
+    log("warmup")
+    for _ in range(args.warmup):
+        ctx.load(inp)
+        ref = ctx.energy("fp64")
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.load(inp)
+        r = ctx.energy("fp64")
+        kms.append(ctx.timing()[0])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    if r["etd"] != ref["etd"] or r["ets"] != ref["ets"]:
+        raise RuntimeError("non-deterministic energy between steps")
+    kernel_s = sum(kms) / len(kms) / 1e3
+    flops = algorithmic_flops(no, nu)
+    items, exec_flops = ctx.work_stats()
+    n_tup = no * (no - 1) * (no - 2) // 6 + no * (no - 1)
+    achieved = flops / kernel_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": ms_step / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"closed-shell (T) E(T)=E[T]+E_ST and v1, no={no} nu={nu}, "
+                               f"{no * (no - 1) * (no - 2) // 6} distinct + {no * (no - 1)} degenerate tuples, "
+                               f"FP64 form, inputs resident in HBM", "no": no, "nu": nu, "seed": seed,
+                   "form": "fp64"},
+        "e_t": r["e"], "etd": r["etd"], "ets": r["ets"],
+        "tuples_per_s": n_tup / (ms_step / 1e3),
+        "kernel_ms": kernel_s * 1e3,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "executed_flops_per_launch": exec_flops, "work_items": items},
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(inp, no, nu, args.cpu_budget, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

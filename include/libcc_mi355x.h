@@ -1,0 +1,145 @@
+/*
+ * libcc_mi355x — closed-shell CCSD(T) triples correction on AMD MI355X (gfx950).
+ *
+ * Drop-in C ABI for the triples path of ryanolson/libcc.  Two layers:
+ *
+ *  1. Legacy Fortran-ABI entry points with the reference's names and argument
+ *     lists: lowercase name + trailing underscore, every argument by reference,
+ *     INTEGER = int64_t (the reference builds with 8-byte default integers,
+ *     src/triples-ijk/Makefile:10), DOUBLE PRECISION = double, arrays are raw
+ *     column-major host pointers.  A Fortran caller links against this library
+ *     unchanged.  As in the reference these return nothing and keep the energies
+ *     in process-global state; unlike the reference they never stop the process:
+ *     a failure is reported by ccx_last_status().
+ *
+ *  2. Context-based entry points (ccx_*) returning an int status (CCX_OK = 0),
+ *     for callers that can hold a handle, use device-resident inputs, or want the
+ *     other arithmetic forms.
+ *
+ * Array layouts are the reference's (Fortran order, 1-based in Fortran terms):
+ *   eh(no), ep(nu), t1(no,nu), t2(nu,nu,no,no), vm(no,nu,no,no), voe(nu,nu,no,no),
+ *   vvvo(nu,nu,nu,no) unpacked, or the distributed layout (nutr, nu*no) with
+ *   nutr = nu*(nu+1)/2 (lower-packed symmetric first pair; column c + nu*(i-1)).
+ */
+#ifndef LIBCC_MI355X_H
+#define LIBCC_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define CCX_OK 0
+#define CCX_ERR_ARG 1      /* bad dimension, index or null pointer           */
+#define CCX_ERR_DEVICE 2   /* no usable GPU or a HIP call failed              */
+#define CCX_ERR_ALLOC 3    /* device or host allocation failed                */
+#define CCX_ERR_STATE 4    /* call out of order (no inputs loaded, no init)   */
+#define CCX_ERR_INPUT 5    /* NaN/Inf in an input array                       */
+#define CCX_ERR_FORM 6     /* arithmetic form not available                   */
+
+/* arithmetic forms of the triples contraction */
+#define CCX_FORM_FP64 0    /* full FP64 (v_mfma_f64_16x16x4_f64)              */
+
+const char *ccx_error_string(int status);
+int ccx_last_status(void);          /* status of the last legacy (void) call    */
+
+/* ---------------------------------------------------------------- context API */
+typedef struct ccx_ctx ccx_ctx;
+
+/* New context for no occupied / nu virtual orbitals on the current HIP device. */
+int ccx_create(int64_t no, int64_t nu, ccx_ctx **ctx);
+int ccx_destroy(ccx_ctx *ctx);
+
+/* Upload inputs from host memory (reference layouts).  vvvo_layout: 0 = unpacked
+ * (nu,nu,nu,no), 1 = distributed packed (nutr, nu*no).  Inputs are checked for
+ * NaN/Inf (CCX_ERR_INPUT). */
+int ccx_load_host(ccx_ctx *ctx, const double *eh, const double *ep, const double *t1,
+                  const double *t2, const double *vm, const double *voe, const double *vvvo,
+                  int vvvo_layout);
+
+/* Same with device pointers (HBM-resident inputs; nothing crosses PCIe). */
+int ccx_load_device(ccx_ctx *ctx, const double *eh, const double *ep, const double *t1,
+                    const double *t2, const double *vm, const double *voe, const double *vvvo,
+                    int vvvo_layout);
+
+/* E[T] (etd), E_ST (ets) and v1(nu,no) over all i>=j>=k tuples in one persistent
+ * launch.  v1 may be NULL; v1_on_device selects whether v1 is a device pointer. */
+int ccx_energy(ccx_ctx *ctx, int form, double *ets, double *etd, double *v1, int v1_on_device);
+
+/* W_ijk(a,b,c) cube (nu^3, column-major, host) of one tuple, 1-based i,j,k, as the
+ * reference's ijk_tuple leaves it in V3 (before the W(a,a,a)=0 step it is identical
+ * off the diagonal). */
+int ccx_tuple_w(ccx_ctx *ctx, int64_t i, int64_t j, int64_t k, double *w);
+
+/* Timing of the last ccx_energy call in milliseconds (HIP events around the
+ * persistent kernel only, and around the whole call). */
+int ccx_last_timing(const ccx_ctx *ctx, double *kernel_ms, double *total_ms);
+
+/* Work items and work-item FLOPs of the persistent launch (for roofline accounting). */
+int ccx_work_stats(const ccx_ctx *ctx, int64_t *items, double *executed_flops);
+
+/* Set the stream (hipStream_t cast to void*) later calls use; NULL = default stream. */
+int ccx_set_stream(ccx_ctx *ctx, void *stream);
+
+/* ---------------------------------------------------------------- legacy Fortran ABI */
+
+/* common_cc_init(nocc, nvir): src/common/common_cc.f90:342-369.  Sets the problem
+ * dimensions and zeroes ets/etd for the calls below. */
+void common_cc_init_(int64_t *nocc, int64_t *nvir);
+
+/* DDI handle adapter: the GAMESS entry receives VVVO as a DDI array handle.  Register
+ * the host array behind a handle first (layout as in ccx_load_host). */
+void ccx_register_vvvo_(int64_t *handle, const double *vvvo, int64_t *layout);
+
+/* cc_triples_acc(eh,ep,v1,t1,t2,v3,t3,vm,voe,vvvo_hnd,gms_ets,gms_etd):
+ * src/triples-ijk/triples.F90:1-12,319-320.  Writes v1(nu,no), gms_ets, gms_etd and the
+ * module energies; like the reference it transposes t1 in place to (nu,no) on return
+ * (triples.F90:311).  v3 and t3 are scratch in the reference and are not touched. */
+void cc_triples_acc_(double *eh, double *ep, double *v1, double *t1, double *t2, double *v3,
+                     double *t3, double *vm, double *voe, int64_t *vvvo_hnd, double *gms_ets,
+                     double *gms_etd);
+
+/* cc_triples(eh,ep,v1,t1,t2,v3,t3,vm,voe,ve_i,ve_j,ve_k): src/triples-ijk/triples_cpu.F90:243-249
+ * (and src/triples-abc/triples.f90:193-199).  VVVO comes from the handle registered for the
+ * current d_vvvo (ccx_set_d_vvvo_); results in the module energies (ccx_get_ets_/etd_). */
+void cc_triples_(double *eh, double *ep, double *v1, double *t1, double *t2, double *v3,
+                 double *t3, double *vm, double *voe, double *ve_i, double *ve_j, double *ve_k);
+
+/* The module variable d_vvvo (src/common/common_cc.f90:10) that cc_triples reads. */
+void ccx_set_d_vvvo_(int64_t *handle);
+
+/* Module accumulators ets / etd (src/common/common_cc.f90:32). */
+double ccx_get_ets_(void);
+double ccx_get_etd_(void);
+
+/* ijk_tuple(i,j,k,t2_i,t2_j,t2_k,vm_ij,vm_ji,vm_ik,vm_ki,vm_jk,vm_kj,ve_i,ve_j,ve_k,v3):
+ * src/triples-ijk/ijk_tuple.f90:34-35.  Forms W_ijk into v3 (nu^3) on the GPU; uses the
+ * dimensions from common_cc_init_.  t2_x are T2(:,:,:,x) blocks, vm_xy VM(:,:,x,y) blocks
+ * and ve_x the fetched VE_x(d,p,q) = VVVO(d,q,p,x) cubes, exactly as the reference passes them. */
+void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const double *t2_j,
+                const double *t2_k, const double *vm_ij, const double *vm_ji, const double *vm_ik,
+                const double *vm_ki, const double *vm_jk, const double *vm_kj, const double *ve_i,
+                const double *ve_j, const double *ve_k, double *v3);
+
+/* trant3 family: in-place permutations of an n^3 cube, new(x,y,z) = old(s(x,y,z)).
+ *   trant3_1_ (x,z,y)  trant3_3_ (z,y,x)  trant3_4_ (y,z,x)  trant3_5_ (z,x,y)
+ * src/common/common_cc.f90:62-156.  trant3_acc_(v,nu,id) and trant3_smp_(v,nu,id) select by
+ * id 1..5: (x,z,y) (y,x,z) (z,y,x) (y,z,x) (z,x,y) (common_cc.f90:240-333,
+ * src/triples-abc/trant3.f:1).  tranmd_23_ swaps indices 2 and 3 of an (n1,n2,n3,n4) array
+ * (common_cc.f90:159-201).  The triples-abc copy also defines a one-argument trant3_1_
+ * (src/triples-abc/ijk_tuple.f90:55); this library exports the two-argument form. */
+void trant3_1_(int64_t *n, double *v);
+void trant3_3_(int64_t *n, double *v);
+void trant3_4_(int64_t *n, double *v);
+void trant3_5_(int64_t *n, double *v);
+void trant3_acc_(double *v, int64_t *nu, int64_t *id);
+void trant3_smp_(double *v, int64_t *nu, int64_t *id);
+void tranmd_23_(double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBCC_MI355X_H */

@@ -1,0 +1,36 @@
+/* Internal: the thin C ABI between the C host layer (ccx_host.c) and the HIP kernels
+ * (ccx_kernels.hip).  Every function returns 0 on success.  Streams and events are
+ * opaque void* (hipStream_t / hipEvent_t). */
+#ifndef CCX_DEV_H
+#define CCX_DEV_H
+
+#include <stddef.h>
+
+int ccx_hip_device_count(int *n);
+int ccx_hip_malloc(void **p, size_t bytes);
+int ccx_hip_free(void *p);
+int ccx_hip_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int ccx_hip_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int ccx_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int ccx_hip_sync(void *stream);
+int ccx_hip_cu_count(int *n);
+const char *ccx_hip_last_error(void);
+int ccx_hip_lds_bytes(void);
+
+int ccx_hip_unpack_vvvo(const double *packed, double *full, int no, int nu, void *stream);
+int ccx_hip_pack(const double *vvvo, const double *t2, const double *vm, const double *voe, double *ax,
+                 double *bx, double *voep, int no, int nu, int nup, int kp, void *stream);
+int ccx_hip_triples_f64(const double *ax, const double *bx, const double *voep, const double *eh,
+                        const double *ep, const int *tuples, const int *blocks, double *e_part, double *v1_part,
+                        double *wout, int no, int nu, int nup, int kp, int ntup, int nblk, int grid,
+                        void *stream);
+int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tuples, const int *blocks,
+                     const double *t1, double *v1t, double *v1, double *out, int no, int nu, int ntup, int nblk,
+                     void *stream);
+
+int ccx_hip_event_create(void **ev);
+int ccx_hip_event_destroy(void *ev);
+int ccx_hip_event_record(void *ev, void *stream);
+int ccx_hip_event_ms(void *start, void *stop, float *ms);
+
+#endif

@@ -1,0 +1,567 @@
+/*
+ * Host layer (C) of libcc_mi355x: argument checking, the device-resident data model,
+ * the tuple / block schedule, and the legacy Fortran-ABI entry points of the
+ * reference's triples path.  All device work goes through the thin ABI in ccx_dev.h.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/libcc_mi355x.h"
+#include "ccx_dev.h"
+
+#define TB 16
+
+struct ccx_ctx {
+    int64_t no, nu, nup, nb, kp;
+    int64_t ntup, nblk;
+    void *stream;
+    int loaded;
+    double *d_ax, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
+    int *d_tuples, *d_blocks;
+    double *d_epart, *d_v1part, *d_v1t, *d_v1, *d_out;
+    int *h_tuples, *h_blocks;
+    double kernel_ms, total_ms;
+    void *ev[4];
+};
+
+const char *ccx_error_string(int s)
+{
+    switch (s) {
+    case CCX_OK: return "ok";
+    case CCX_ERR_ARG: return "invalid argument";
+    case CCX_ERR_DEVICE: return "GPU unavailable or HIP call failed";
+    case CCX_ERR_ALLOC: return "allocation failed";
+    case CCX_ERR_STATE: return "call out of order";
+    case CCX_ERR_INPUT: return "non-finite value in an input array";
+    case CCX_ERR_FORM: return "arithmetic form not available";
+    default: return "unknown status";
+    }
+}
+
+/* ------------------------------------------------------------------ schedule tables */
+
+/* Occupied tuples in the reference driver's order: distinct i>j>k with i outermost
+ * (DDCC_T_TASK, src/triples-ijk/common_functions.f:62-86), then for each i>j the
+ * (i,j,j) and (i,i,j) tuples (src/triples-ijk/triples.F90:237-300).  0-based; the 4th
+ * entry is 1 for a distinct tuple and 2 for a degenerate one (energy weight 1/2,
+ * src/triples-ijk/t1wt3.f90:188-192). */
+static int64_t build_tuples(int64_t no, int *out)
+{
+    int64_t n = 0;
+    for (int64_t i = 0; i < no; ++i)
+        for (int64_t j = 0; j < i; ++j)
+            for (int64_t k = 0; k < j; ++k) {
+                if (out) { out[4 * n] = (int)i; out[4 * n + 1] = (int)j; out[4 * n + 2] = (int)k; out[4 * n + 3] = 1; }
+                ++n;
+            }
+    for (int64_t i = 0; i < no; ++i)
+        for (int64_t j = 0; j < i; ++j) {
+            if (out) {
+                out[4 * n] = (int)i; out[4 * n + 1] = (int)j; out[4 * n + 2] = (int)j; out[4 * n + 3] = 2;
+                out[4 * n + 4] = (int)i; out[4 * n + 5] = (int)i; out[4 * n + 6] = (int)j; out[4 * n + 7] = 2;
+            }
+            n += 2;
+        }
+    return n;
+}
+
+/* Virtual 16^3 block orbits A<=B<=C; 4th entry = how many positional permutations
+ * map the block onto itself (1, 2 or 6), the factor by which the orbit covers the cube. */
+static int64_t build_blocks(int64_t nb, int *out)
+{
+    int64_t n = 0;
+    for (int64_t a = 0; a < nb; ++a)
+        for (int64_t b = a; b < nb; ++b)
+            for (int64_t c = b; c < nb; ++c) {
+                if (out) {
+                    int m = (a == b && b == c) ? 6 : (a == b || b == c) ? 2 : 1;
+                    out[4 * n] = (int)a; out[4 * n + 1] = (int)b; out[4 * n + 2] = (int)c; out[4 * n + 3] = m;
+                }
+                ++n;
+            }
+    return n;
+}
+
+/* ------------------------------------------------------------------ context */
+
+static void free_dev(ccx_ctx *c)
+{
+    double **d[] = {&c->d_ax, &c->d_bx, &c->d_voep, &c->d_eh, &c->d_ep, &c->d_t1,
+                    &c->d_epart, &c->d_v1part, &c->d_v1t, &c->d_v1, &c->d_out};
+    for (size_t q = 0; q < sizeof(d) / sizeof(d[0]); ++q) {
+        ccx_hip_free(*d[q]);
+        *d[q] = NULL;
+    }
+    ccx_hip_free(c->d_tuples);
+    ccx_hip_free(c->d_blocks);
+    c->d_tuples = c->d_blocks = NULL;
+    c->loaded = 0;
+}
+
+int ccx_create(int64_t no, int64_t nu, ccx_ctx **out)
+{
+    if (!out) return CCX_ERR_ARG;
+    *out = NULL;
+    if (no < 1 || nu < 1 || no > 2048 || nu > 4096) return CCX_ERR_ARG;
+    int ndev = 0;
+    if (ccx_hip_device_count(&ndev) != 0 || ndev < 1) return CCX_ERR_DEVICE;
+    ccx_ctx *c = (ccx_ctx *)calloc(1, sizeof(ccx_ctx));
+    if (!c) return CCX_ERR_ALLOC;
+    c->no = no;
+    c->nu = nu;
+    c->nup = (nu + TB - 1) / TB * TB;
+    c->nb = c->nup / TB;
+    c->kp = (nu + no + 3) / 4 * 4;
+    c->ntup = build_tuples(no, NULL);
+    c->nblk = build_blocks(c->nb, NULL);
+    c->h_tuples = (int *)malloc(sizeof(int) * 4 * (size_t)(c->ntup > 0 ? c->ntup : 1));
+    c->h_blocks = (int *)malloc(sizeof(int) * 4 * (size_t)c->nblk);
+    if (!c->h_tuples || !c->h_blocks) { free(c->h_tuples); free(c->h_blocks); free(c); return CCX_ERR_ALLOC; }
+    build_tuples(no, c->h_tuples);
+    build_blocks(c->nb, c->h_blocks);
+    for (int q = 0; q < 4; ++q)
+        if (ccx_hip_event_create(&c->ev[q]) != 0) { free(c->h_tuples); free(c->h_blocks); free(c); return CCX_ERR_DEVICE; }
+    *out = c;
+    return CCX_OK;
+}
+
+int ccx_destroy(ccx_ctx *c)
+{
+    if (!c) return CCX_ERR_ARG;
+    free_dev(c);
+    for (int q = 0; q < 4; ++q) ccx_hip_event_destroy(c->ev[q]);
+    free(c->h_tuples);
+    free(c->h_blocks);
+    free(c);
+    return CCX_OK;
+}
+
+int ccx_set_stream(ccx_ctx *c, void *stream)
+{
+    if (!c) return CCX_ERR_ARG;
+    c->stream = stream;
+    return CCX_OK;
+}
+
+static int finite_all(const double *p, size_t n)
+{
+    for (size_t q = 0; q < n; ++q)
+        if (!isfinite(p[q])) return 0;
+    return 1;
+}
+
+#define DALLOC(ptr, count)                                                             \
+    do {                                                                               \
+        if (ccx_hip_malloc((void **)&(ptr), sizeof(*(ptr)) * (size_t)(count)) != 0) { \
+            rc = CCX_ERR_ALLOC;                                                        \
+            goto fail;                                                                 \
+        }                                                                              \
+    } while (0)
+
+/* Build the packed operand slabs and small arrays from device-resident raw inputs. */
+static int build_from_device(ccx_ctx *c, const double *d_eh, const double *d_ep_raw, const double *d_t1,
+                             const double *d_t2, const double *d_vm, const double *d_voe,
+                             const double *d_vvvo_full)
+{
+    int rc = CCX_OK;
+    const int64_t no = c->no, nu = c->nu, nup = c->nup, nb = c->nb, kp = c->kp;
+    const int64_t nitems = c->ntup * c->nblk;
+    double *h_ep = NULL;
+    if (c->d_ax) goto allocated; /* buffers depend only on the dimensions: reuse */
+    DALLOC(c->d_ax, no * nb * nb * kp * 256);
+    DALLOC(c->d_bx, no * no * kp * nup);
+    DALLOC(c->d_voep, no * no * nup * nup);
+    DALLOC(c->d_eh, no);
+    DALLOC(c->d_ep, nup);
+    DALLOC(c->d_t1, no * nu);
+    DALLOC(c->d_tuples, 4 * (c->ntup > 0 ? c->ntup : 1));
+    DALLOC(c->d_blocks, 4 * c->nblk);
+    DALLOC(c->d_epart, nitems > 0 ? nitems : 1);
+    DALLOC(c->d_v1part, (nitems > 0 ? nitems : 1) * 144);
+    DALLOC(c->d_v1t, (c->ntup > 0 ? c->ntup : 1) * nu * 3);
+    DALLOC(c->d_v1, no * nu);
+    DALLOC(c->d_out, 4);
+allocated:
+    c->loaded = 0;
+    if (ccx_hip_pack(d_vvvo_full, d_t2, d_vm, d_voe, c->d_ax, c->d_bx, c->d_voep, (int)no, (int)nu, (int)nup,
+                     (int)kp, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    /* eps_virtual padded to nup with finite values (padded elements carry W = 0) */
+    h_ep = (double *)malloc(sizeof(double) * (size_t)nup);
+    if (!h_ep) { rc = CCX_ERR_ALLOC; goto fail; }
+    if (ccx_hip_memcpy_d2h(h_ep, d_ep_raw, sizeof(double) * (size_t)nu, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    for (int64_t a = nu; a < nup; ++a) h_ep[a] = 1.0e3;
+    if (ccx_hip_memcpy_h2d(c->d_ep, h_ep, sizeof(double) * (size_t)nup, c->stream) != 0 ||
+        ccx_hip_memcpy_d2d(c->d_eh, d_eh, sizeof(double) * (size_t)no, c->stream) != 0 ||
+        ccx_hip_memcpy_d2d(c->d_t1, d_t1, sizeof(double) * (size_t)(no * nu), c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(c->d_tuples, c->h_tuples, sizeof(int) * 4 * (size_t)c->ntup, c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(c->d_blocks, c->h_blocks, sizeof(int) * 4 * (size_t)c->nblk, c->stream) != 0 ||
+        ccx_hip_sync(c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    free(h_ep);
+    c->loaded = 1;
+    return CCX_OK;
+fail:
+    free(h_ep);
+    free_dev(c);
+    return rc;
+}
+
+int ccx_load_device(ccx_ctx *c, const double *eh, const double *ep, const double *t1, const double *t2,
+                    const double *vm, const double *voe, const double *vvvo, int vvvo_layout)
+{
+    if (!c || !eh || !ep || !t1 || !t2 || !vm || !voe || !vvvo) return CCX_ERR_ARG;
+    if (vvvo_layout != 0 && vvvo_layout != 1) return CCX_ERR_ARG;
+    if (vvvo_layout == 0) return build_from_device(c, eh, ep, t1, t2, vm, voe, vvvo);
+    double *full = NULL;
+    if (ccx_hip_malloc((void **)&full, sizeof(double) * (size_t)(c->nu * c->nu * c->nu * c->no)) != 0)
+        return CCX_ERR_ALLOC;
+    int rc = ccx_hip_unpack_vvvo(vvvo, full, (int)c->no, (int)c->nu, c->stream) != 0 ? CCX_ERR_DEVICE : CCX_OK;
+    if (rc == CCX_OK) rc = build_from_device(c, eh, ep, t1, t2, vm, voe, full);
+    ccx_hip_sync(c->stream);
+    ccx_hip_free(full);
+    return rc;
+}
+
+int ccx_load_host(ccx_ctx *c, const double *eh, const double *ep, const double *t1, const double *t2,
+                  const double *vm, const double *voe, const double *vvvo, int vvvo_layout)
+{
+    if (!c || !eh || !ep || !t1 || !t2 || !vm || !voe || !vvvo) return CCX_ERR_ARG;
+    if (vvvo_layout != 0 && vvvo_layout != 1) return CCX_ERR_ARG;
+    const int64_t no = c->no, nu = c->nu;
+    const size_t n_t2 = (size_t)(nu * nu * no * no), n_vm = (size_t)(no * nu * no * no);
+    const size_t n_vvvo = vvvo_layout == 0 ? (size_t)(nu * nu * nu * no) : (size_t)(nu * (nu + 1) / 2 * nu * no);
+    if (!finite_all(eh, (size_t)no) || !finite_all(ep, (size_t)nu) || !finite_all(t1, (size_t)(no * nu)) ||
+        !finite_all(t2, n_t2) || !finite_all(vm, n_vm) || !finite_all(voe, n_t2) || !finite_all(vvvo, n_vvvo))
+        return CCX_ERR_INPUT;
+    int rc = CCX_OK;
+    double *d_eh = NULL, *d_ep = NULL, *d_t1 = NULL, *d_t2 = NULL, *d_vm = NULL, *d_voe = NULL, *d_vvvo = NULL;
+    DALLOC(d_eh, no);
+    DALLOC(d_ep, nu);
+    DALLOC(d_t1, no * nu);
+    DALLOC(d_t2, n_t2);
+    DALLOC(d_vm, n_vm);
+    DALLOC(d_voe, n_t2);
+    DALLOC(d_vvvo, n_vvvo);
+    if (ccx_hip_memcpy_h2d(d_eh, eh, sizeof(double) * (size_t)no, c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(d_ep, ep, sizeof(double) * (size_t)nu, c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(d_t1, t1, sizeof(double) * (size_t)(no * nu), c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(d_t2, t2, sizeof(double) * n_t2, c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(d_vm, vm, sizeof(double) * n_vm, c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(d_voe, voe, sizeof(double) * n_t2, c->stream) != 0 ||
+        ccx_hip_memcpy_h2d(d_vvvo, vvvo, sizeof(double) * n_vvvo, c->stream) != 0) {
+        rc = CCX_ERR_DEVICE;
+        goto fail;
+    }
+    rc = ccx_load_device(c, d_eh, d_ep, d_t1, d_t2, d_vm, d_voe, d_vvvo, vvvo_layout);
+    ccx_hip_sync(c->stream);
+fail:
+    ccx_hip_free(d_eh); ccx_hip_free(d_ep); ccx_hip_free(d_t1); ccx_hip_free(d_t2);
+    ccx_hip_free(d_vm); ccx_hip_free(d_voe); ccx_hip_free(d_vvvo);
+    return rc;
+}
+
+static int grid_size(const ccx_ctx *c, int64_t nitems)
+{
+    int cus = 256;
+    if (ccx_hip_cu_count(&cus) != 0 || cus < 1) cus = 256;
+    int64_t g = cus;
+    if (g > nitems) g = nitems;
+    return (int)(g < 1 ? 1 : g);
+    (void)c;
+}
+
+int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v1_on_device)
+{
+    if (!c) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    if (form != CCX_FORM_FP64) return CCX_ERR_FORM;
+    const int64_t nitems = c->ntup * c->nblk;
+    double host_out[2] = {0.0, 0.0};
+    if (ccx_hip_event_record(c->ev[0], c->stream) != 0) return CCX_ERR_DEVICE;
+    if (ccx_hip_event_record(c->ev[1], c->stream) != 0) return CCX_ERR_DEVICE;
+    if (nitems > 0 &&
+        ccx_hip_triples_f64(c->d_ax, c->d_bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
+                            c->d_v1part, NULL, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, (int)c->ntup,
+                            (int)c->nblk, grid_size(c, nitems), c->stream) != 0)
+        return CCX_ERR_DEVICE;
+    if (ccx_hip_event_record(c->ev[2], c->stream) != 0) return CCX_ERR_DEVICE;
+    if (ccx_hip_finalize(c->d_epart, c->d_v1part, c->d_tuples, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,
+                         (int)c->no, (int)c->nu, (int)c->ntup, (int)c->nblk, c->stream) != 0)
+        return CCX_ERR_DEVICE;
+    if (ccx_hip_event_record(c->ev[3], c->stream) != 0) return CCX_ERR_DEVICE;
+    if (ccx_hip_memcpy_d2h(host_out, c->d_out, sizeof(double) * 2, c->stream) != 0) return CCX_ERR_DEVICE;
+    if (v1) {
+        const size_t bytes = sizeof(double) * (size_t)(c->no * c->nu);
+        int r = v1_on_device ? ccx_hip_memcpy_d2d(v1, c->d_v1, bytes, c->stream)
+                             : ccx_hip_memcpy_d2h(v1, c->d_v1, bytes, c->stream);
+        if (r != 0 || ccx_hip_sync(c->stream) != 0) return CCX_ERR_DEVICE;
+    }
+    float k_ms = 0.0f, t_ms = 0.0f;
+    if (ccx_hip_event_ms(c->ev[1], c->ev[2], &k_ms) != 0 || ccx_hip_event_ms(c->ev[0], c->ev[3], &t_ms) != 0)
+        return CCX_ERR_DEVICE;
+    c->kernel_ms = k_ms;
+    c->total_ms = t_ms;
+    if (etd) *etd = host_out[0];
+    if (ets) *ets = host_out[1];
+    return CCX_OK;
+}
+
+int ccx_tuple_w(ccx_ctx *c, int64_t i, int64_t j, int64_t k, double *w)
+{
+    if (!c || !w) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    if (i < 1 || j < 1 || k < 1 || i > c->no || j > c->no || k > c->no) return CCX_ERR_ARG;
+    const size_t nu3 = (size_t)(c->nu * c->nu * c->nu);
+    int rc = CCX_OK;
+    int *d_tup = NULL;
+    double *d_w = NULL, *d_ep1 = NULL, *d_vp = NULL;
+    int tup[4] = {(int)(i - 1), (int)(j - 1), (int)(k - 1), 1};
+    DALLOC(d_tup, 4);
+    DALLOC(d_w, nu3);
+    DALLOC(d_ep1, c->nblk);
+    DALLOC(d_vp, c->nblk * 144);
+    if (ccx_hip_memcpy_h2d(d_tup, tup, sizeof(tup), c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    if (ccx_hip_triples_f64(c->d_ax, c->d_bx, c->d_voep, c->d_eh, c->d_ep, d_tup, c->d_blocks, d_ep1, d_vp, d_w,
+                            (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, 1, (int)c->nblk,
+                            grid_size(c, c->nblk), c->stream) != 0 ||
+        ccx_hip_memcpy_d2h(w, d_w, sizeof(double) * nu3, c->stream) != 0)
+        rc = CCX_ERR_DEVICE;
+fail:
+    ccx_hip_free(d_tup); ccx_hip_free(d_w); ccx_hip_free(d_ep1); ccx_hip_free(d_vp);
+    return rc;
+}
+
+int ccx_last_timing(const ccx_ctx *c, double *kernel_ms, double *total_ms)
+{
+    if (!c) return CCX_ERR_ARG;
+    if (kernel_ms) *kernel_ms = c->kernel_ms;
+    if (total_ms) *total_ms = c->total_ms;
+    return CCX_OK;
+}
+
+int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
+{
+    if (!c) return CCX_ERR_ARG;
+    const int64_t nitems = c->ntup * c->nblk;
+    if (items) *items = nitems;
+    /* per item: 36 terms x 16^3 outputs x kp MACs */
+    if (executed_flops) *executed_flops = (double)nitems * 36.0 * 4096.0 * 2.0 * (double)c->kp;
+    return CCX_OK;
+}
+
+/* ------------------------------------------------------------------ legacy Fortran ABI */
+
+static int64_t g_no = 0, g_nu = 0;
+static double g_ets = 0.0, g_etd = 0.0;
+static int g_status = CCX_OK;
+static int64_t g_d_vvvo = 0;
+static ccx_ctx *g_ctx = NULL;
+
+#define NREG 16
+static struct { int64_t handle; const double *ptr; int64_t layout; } g_reg[NREG];
+static int g_nreg = 0;
+
+int ccx_last_status(void) { return g_status; }
+
+void common_cc_init_(int64_t *nocc, int64_t *nvir)
+{
+    if (!nocc || !nvir || *nocc < 1 || *nvir < 1) { g_status = CCX_ERR_ARG; return; }
+    g_no = *nocc;
+    g_nu = *nvir;
+    g_ets = 0.0;
+    g_etd = 0.0;
+    g_status = CCX_OK;
+}
+
+void ccx_register_vvvo_(int64_t *handle, const double *vvvo, int64_t *layout)
+{
+    if (!handle || !vvvo || !layout || (*layout != 0 && *layout != 1)) { g_status = CCX_ERR_ARG; return; }
+    for (int q = 0; q < g_nreg; ++q)
+        if (g_reg[q].handle == *handle) { g_reg[q].ptr = vvvo; g_reg[q].layout = *layout; g_status = CCX_OK; return; }
+    if (g_nreg == NREG) { g_status = CCX_ERR_ALLOC; return; }
+    g_reg[g_nreg].handle = *handle;
+    g_reg[g_nreg].ptr = vvvo;
+    g_reg[g_nreg].layout = *layout;
+    ++g_nreg;
+    g_status = CCX_OK;
+}
+
+void ccx_set_d_vvvo_(int64_t *handle)
+{
+    if (!handle) { g_status = CCX_ERR_ARG; return; }
+    g_d_vvvo = *handle;
+    g_status = CCX_OK;
+}
+
+double ccx_get_ets_(void) { return g_ets; }
+double ccx_get_etd_(void) { return g_etd; }
+
+static int legacy_ctx(void)
+{
+    if (g_no < 1 || g_nu < 1) return CCX_ERR_STATE;
+    if (g_ctx && (g_ctx->no != g_no || g_ctx->nu != g_nu)) {
+        ccx_destroy(g_ctx);
+        g_ctx = NULL;
+    }
+    if (!g_ctx) return ccx_create(g_no, g_nu, &g_ctx);
+    return CCX_OK;
+}
+
+/* TRPOSE(t1,tmp,no,nu,1): t1(no,nu) -> t1(nu,no) in place (src/triples-ijk/common_functions.f:3-41) */
+static void transpose_inplace(double *a, int64_t n, int64_t m)
+{
+    double *b = (double *)malloc(sizeof(double) * (size_t)(n * m));
+    if (!b) return;
+    for (int64_t j = 0; j < m; ++j)
+        for (int64_t i = 0; i < n; ++i) b[j + m * i] = a[i + n * j];
+    memcpy(a, b, sizeof(double) * (size_t)(n * m));
+    free(b);
+}
+
+static void legacy_triples(int64_t handle, double *eh, double *ep, double *v1, double *t1, double *t2,
+                           double *vm, double *voe, double *gms_ets, double *gms_etd)
+{
+    const double *vvvo = NULL;
+    int64_t layout = 0;
+    for (int q = 0; q < g_nreg; ++q)
+        if (g_reg[q].handle == handle) { vvvo = g_reg[q].ptr; layout = g_reg[q].layout; }
+    if (!vvvo) { g_status = CCX_ERR_STATE; return; }
+    if (!eh || !ep || !v1 || !t1 || !t2 || !vm || !voe) { g_status = CCX_ERR_ARG; return; }
+    int rc = legacy_ctx();
+    if (rc == CCX_OK) rc = ccx_load_host(g_ctx, eh, ep, t1, t2, vm, voe, vvvo, (int)layout);
+    double ets = 0.0, etd = 0.0;
+    if (rc == CCX_OK) rc = ccx_energy(g_ctx, CCX_FORM_FP64, &ets, &etd, v1, 0);
+    g_status = rc;
+    if (rc != CCX_OK) return;
+    g_ets = ets;
+    g_etd = etd;
+    if (gms_ets) *gms_ets = ets;
+    if (gms_etd) *gms_etd = etd;
+    transpose_inplace(t1, g_no, g_nu);
+}
+
+void cc_triples_acc_(double *eh, double *ep, double *v1, double *t1, double *t2, double *v3, double *t3,
+                     double *vm, double *voe, int64_t *vvvo_hnd, double *gms_ets, double *gms_etd)
+{
+    (void)v3;
+    (void)t3;
+    if (!vvvo_hnd) { g_status = CCX_ERR_ARG; return; }
+    legacy_triples(*vvvo_hnd, eh, ep, v1, t1, t2, vm, voe, gms_ets, gms_etd);
+}
+
+void cc_triples_(double *eh, double *ep, double *v1, double *t1, double *t2, double *v3, double *t3, double *vm,
+                 double *voe, double *ve_i, double *ve_j, double *ve_k)
+{
+    (void)v3; (void)t3; (void)ve_i; (void)ve_j; (void)ve_k;
+    legacy_triples(g_d_vvvo, eh, ep, v1, t1, t2, vm, voe, NULL, NULL);
+}
+
+void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const double *t2_j, const double *t2_k,
+                const double *vm_ij, const double *vm_ji, const double *vm_ik, const double *vm_ki,
+                const double *vm_jk, const double *vm_kj, const double *ve_i, const double *ve_j,
+                const double *ve_k, double *v3)
+{
+    if (!i || !j || !k || !t2_i || !t2_j || !t2_k || !vm_ij || !vm_ji || !vm_ik || !vm_ki || !vm_jk || !vm_kj ||
+        !ve_i || !ve_j || !ve_k || !v3) { g_status = CCX_ERR_ARG; return; }
+    const int64_t no = g_no, nu = g_nu, nu2 = nu * nu;
+    if (no < 1 || nu < 1) { g_status = CCX_ERR_STATE; return; }
+    const int64_t ii = *i - 1, jj = *j - 1, kk = *k - 1;
+    if (ii < 0 || jj < 0 || kk < 0 || ii >= no || jj >= no || kk >= no) { g_status = CCX_ERR_ARG; return; }
+    /* Assemble full-size arrays holding only the slices the tuple reads; the rest is zero. */
+    const size_t n_t2 = (size_t)(nu2 * no * no), n_vm = (size_t)(no * nu * no * no);
+    double *t2 = (double *)calloc(n_t2, sizeof(double)), *vm = (double *)calloc(n_vm, sizeof(double));
+    double *voe = (double *)calloc(n_t2, sizeof(double));
+    double *vvvo = (double *)calloc((size_t)(nu2 * nu * no), sizeof(double));
+    double *eh = (double *)calloc((size_t)no, sizeof(double)), *ep = (double *)calloc((size_t)nu, sizeof(double));
+    double *t1 = (double *)calloc((size_t)(no * nu), sizeof(double));
+    ccx_ctx *c = NULL;
+    int rc = CCX_OK;
+    if (!t2 || !vm || !voe || !vvvo || !eh || !ep || !t1) { rc = CCX_ERR_ALLOC; goto done; }
+    {
+        const int64_t xs[3] = {ii, jj, kk};
+        const double *t2s[3] = {t2_i, t2_j, t2_k};
+        const double *ves[3] = {ve_i, ve_j, ve_k};
+        for (int p = 0; p < 3; ++p) {
+            memcpy(t2 + (size_t)(nu2 * no * xs[p]), t2s[p], sizeof(double) * (size_t)(nu2 * no));
+            /* VVVO(d,q,p,x) = VE_x(d,p,q) */
+            double *dst = vvvo + (size_t)(nu2 * nu * xs[p]);
+            for (int64_t q = 0; q < nu; ++q)
+                for (int64_t pp = 0; pp < nu; ++pp)
+                    for (int64_t d = 0; d < nu; ++d) dst[d + nu * (q + nu * pp)] = ves[p][d + nu * (pp + nu * q)];
+        }
+        const double *vms[6] = {vm_ij, vm_ji, vm_ik, vm_ki, vm_jk, vm_kj};
+        const int64_t vx[6] = {ii, jj, ii, kk, jj, kk}, vy[6] = {jj, ii, kk, ii, kk, jj};
+        for (int p = 0; p < 6; ++p)
+            memcpy(vm + (size_t)(no * nu * (vx[p] + no * vy[p])), vms[p], sizeof(double) * (size_t)(no * nu));
+    }
+    rc = ccx_create(no, nu, &c);
+    if (rc == CCX_OK) rc = ccx_load_host(c, eh, ep, t1, t2, vm, voe, vvvo, 0);
+    if (rc == CCX_OK) rc = ccx_tuple_w(c, *i, *j, *k, v3);
+    if (rc == CCX_OK)
+        for (int64_t a = 0; a < nu; ++a) v3[a + nu * (a + nu * a)] = 0.0; /* zerot3 */
+done:
+    if (c) ccx_destroy(c);
+    free(t2); free(vm); free(voe); free(vvvo); free(eh); free(ep); free(t1);
+    g_status = rc;
+}
+
+/* ------------------------------------------------------------------ permutations (host) */
+
+/* new(x,y,z) = old(s(x,y,z)); id 1 (x,z,y), 2 (y,x,z), 3 (z,y,x), 4 (y,z,x), 5 (z,x,y) */
+static int cube_perm(double *v, int64_t n, int id)
+{
+    if (!v || n < 0 || id < 1 || id > 5) return CCX_ERR_ARG;
+    const size_t n3 = (size_t)(n * n * n);
+    double *old = (double *)malloc(sizeof(double) * (n3 ? n3 : 1));
+    if (!old) return CCX_ERR_ALLOC;
+    memcpy(old, v, sizeof(double) * n3);
+    for (int64_t z = 0; z < n; ++z)
+        for (int64_t y = 0; y < n; ++y)
+            for (int64_t x = 0; x < n; ++x) {
+                int64_t sx, sy, sz;
+                switch (id) {
+                case 1: sx = x; sy = z; sz = y; break;
+                case 2: sx = y; sy = x; sz = z; break;
+                case 3: sx = z; sy = y; sz = x; break;
+                case 4: sx = y; sy = z; sz = x; break;
+                default: sx = z; sy = x; sz = y; break;
+                }
+                v[x + n * (y + n * z)] = old[sx + n * (sy + n * sz)];
+            }
+    free(old);
+    return CCX_OK;
+}
+
+void trant3_1_(int64_t *n, double *v) { g_status = n ? cube_perm(v, *n, 1) : CCX_ERR_ARG; }
+void trant3_3_(int64_t *n, double *v) { g_status = n ? cube_perm(v, *n, 3) : CCX_ERR_ARG; }
+void trant3_4_(int64_t *n, double *v) { g_status = n ? cube_perm(v, *n, 4) : CCX_ERR_ARG; }
+void trant3_5_(int64_t *n, double *v) { g_status = n ? cube_perm(v, *n, 5) : CCX_ERR_ARG; }
+void trant3_acc_(double *v, int64_t *nu, int64_t *id)
+{
+    g_status = (nu && id) ? cube_perm(v, *nu, (int)*id) : CCX_ERR_ARG;
+}
+void trant3_smp_(double *v, int64_t *nu, int64_t *id)
+{
+    g_status = (nu && id) ? cube_perm(v, *nu, (int)*id) : CCX_ERR_ARG;
+}
+
+void tranmd_23_(double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4)
+{
+    if (!a || !n1 || !n2 || !n3 || !n4 || *n2 != *n3 || *n1 < 0 || *n2 < 0 || *n4 < 0) {
+        g_status = CCX_ERR_ARG;
+        return;
+    }
+    const int64_t N1 = *n1, N2 = *n2, N4 = *n4;
+    for (int64_t l = 0; l < N4; ++l)
+        for (int64_t j = 0; j < N2; ++j)
+            for (int64_t k = 0; k < j; ++k)
+                for (int64_t i = 0; i < N1; ++i) {
+                    double *p = a + i + N1 * (j + N2 * (k + N2 * l));
+                    double *q = a + i + N1 * (k + N2 * (j + N2 * l));
+                    double x = *p;
+                    *p = *q;
+                    *q = x;
+                }
+    g_status = CCX_OK;
+}

@@ -1,0 +1,117 @@
+"""Python mirror of the reference's triples driver interface over libcc_mi355x.so.
+
+``cc_triples(eh, ep, t1, t2, vm, voe, vvvo)`` mirrors ``cc_triples_acc``
+(src/triples-ijk/triples.F90:1-12,319-320): same arrays, same layouts, returns
+(ets, etd, v1) with v1(nu,no).  Unlike the Fortran routine it does not transpose
+``t1`` in place (the legacy C entry ``cc_triples_acc_`` does, for drop-in
+fidelity).  ``TriplesContext`` exposes the context API: host or device-resident
+inputs (numpy arrays or torch tensors on the GPU), FP64 energies, per-tuple W.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import CcxError, lib
+
+FORMS = {"fp64": 0}
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _host(a) -> np.ndarray:
+    a = np.asarray(a, dtype=np.float64)
+    return np.asfortranarray(a)
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise CcxError(status, what)
+
+
+class TriplesContext:
+    """A device-resident (T) problem of no occupied / nu virtual orbitals."""
+
+    def __init__(self, no: int, nu: int):
+        self.no, self.nu = int(no), int(nu)
+        self._ctx = ctypes.c_void_p()
+        _check(lib().ccx_create(self.no, self.nu, ctypes.byref(self._ctx)), "ccx_create")
+        self._keep = None
+
+    def close(self) -> None:
+        if self._ctx:
+            lib().ccx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, inp: dict, packed_vvvo: bool = False) -> None:
+        """Upload inputs in the reference layouts (numpy, host) or use torch GPU tensors in place."""
+        keys = ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")
+        first = inp["vvvo"]
+        layout = 1 if packed_vvvo else 0
+        if hasattr(first, "is_cuda") and first.is_cuda:
+            ptrs = []
+            for k in keys:
+                t = inp[k]
+                if t.dtype != __import__("torch").float64:
+                    raise TypeError(f"{k} must be float64")
+                ptrs.append(ctypes.c_void_p(t.data_ptr()))
+            self._keep = [inp[k] for k in keys]
+            _check(lib().ccx_load_device(self._ctx, *ptrs, layout), "ccx_load_device")
+        else:
+            arrs = [_host(inp[k]) for k in keys]
+            _check(lib().ccx_load_host(self._ctx, *[a.ctypes.data_as(_dp) for a in arrs], layout),
+                   "ccx_load_host")
+
+    def energy(self, form: str = "fp64", v1_device=None) -> dict:
+        ets, etd = ctypes.c_double(), ctypes.c_double()
+        if v1_device is not None:
+            _check(lib().ccx_energy(self._ctx, FORMS[form], ctypes.byref(ets), ctypes.byref(etd),
+                                    ctypes.c_void_p(v1_device.data_ptr()), 1), "ccx_energy")
+            v1 = v1_device
+        else:
+            v1 = np.zeros((self.nu, self.no), order="F")
+            _check(lib().ccx_energy(self._ctx, FORMS[form], ctypes.byref(ets), ctypes.byref(etd),
+                                    ctypes.c_void_p(v1.ctypes.data), 0), "ccx_energy")
+        return {"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value, "v1": v1}
+
+    def tuple_w(self, i: int, j: int, k: int) -> np.ndarray:
+        """W_ijk cube for a 1-based tuple (no W(a,a,a)=0 step)."""
+        w = np.zeros((self.nu, self.nu, self.nu), order="F")
+        _check(lib().ccx_tuple_w(self._ctx, i, j, k, w.ctypes.data_as(_dp)), "ccx_tuple_w")
+        return w
+
+    def timing(self) -> tuple:
+        k, t = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ccx_last_timing(self._ctx, ctypes.byref(k), ctypes.byref(t)), "ccx_last_timing")
+        return k.value, t.value
+
+    def work_stats(self) -> tuple:
+        n, f = ctypes.c_int64(), ctypes.c_double()
+        _check(lib().ccx_work_stats(self._ctx, ctypes.byref(n), ctypes.byref(f)), "ccx_work_stats")
+        return n.value, f.value
+
+
+def cc_triples(eh, ep, t1, t2, vm, voe, vvvo, form: str = "fp64", packed_vvvo: bool = False) -> tuple:
+    """(ets, etd, v1) for the given reference-layout arrays; see module docstring."""
+    no, nu = len(eh), len(ep)
+    ctx = TriplesContext(no, nu)
+    try:
+        ctx.load({"eh": eh, "ep": ep, "t1": t1, "t2": t2, "vm": vm, "voe": voe, "vvvo": vvvo},
+                 packed_vvvo=packed_vvvo)
+        r = ctx.energy(form)
+    finally:
+        ctx.close()
+    return r["ets"], r["etd"], r["v1"]
+
+
+def algorithmic_flops(no: int, nu: int) -> float:
+    """GEMM FLOPs of the reference decomposition: 12 nu^3 (no+nu) per distinct tuple and
+    6 nu^3 (no+nu) per degenerate one (src/triples-ijk/ijk_tuple.f90:19-28) = 2 (no^3-no) nu^3 (no+nu)."""
+    return 2.0 * (no ** 3 - no) * nu ** 3 * (no + nu)

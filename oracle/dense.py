@@ -1,0 +1,131 @@
+"""Dense closed-form oracle for the closed-shell (T) triples path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg as the checker; never part of the product path.
+
+Everything here is O(no^3 nu^3) memory and meant for small shapes.  Arrays use
+the reference layouts produced by ``libcc_amd.synth`` with 0-based indices:
+``vvvo[a,b,c,i] = VVVO(a+1,b+1,c+1,i+1)`` and so on.
+
+* ``w_dense``     the closed form of W_ijk(a,b,c) that the 12 DGEMMs and trant3
+                  steps of ``ijk_tuple`` build (src/triples-ijk/ijk_tuple.f90:56-84):
+                  W = sum over the 6 simultaneous permutations of the pairs
+                  (i,a),(j,b),(k,c) of
+                  [ sum_d VVVO(d,b,a,i) T2(c,d,j,k) - sum_l T2(a,b,l,i) VM(l,c,k,j) ].
+* ``energies``    E[T] (etd), E_ST (ets), v1 as t1wt3_ijk / T3SQUA / DRT1WT3*
+                  accumulate them (src/triples-ijk/t1wt3.f90:53-102,
+                  src/triples-ijk/t3squa_acc.f:18-56, src/triples-ijk/ddcc_aux.f:79-171)
+                  and the driver finishes them (src/triples-ijk/triples.F90:310-313).
+* ``textbook``    an independent evaluation of the closed-shell (T) energy straight
+                  from the integrals (pq|rs), t_i^a and t_ij^ab, in the form of
+                  Rendell, Lee & Komornicki, Chem. Phys. Lett. 178, 462 (1991):
+                  E(T) = sum_{ijk,abc} (4W_abc + W_bca + W_cab)(V_abc - V_cba) / (3 D),
+                  V = W + t_i^a (jb|kc) + t_j^b (ia|kc) + t_k^c (ia|jb).
+                  It shares no index conventions with the reference layouts and so
+                  pins the layout mapping and the weights of the reference path.
+"""
+
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+_PERMS = list(itertools.permutations(range(3)))
+
+
+def _sym6(p: np.ndarray) -> np.ndarray:
+    """Sum of the 6 simultaneous permutations of axis pairs (0,3),(1,4),(2,5)."""
+    out = np.zeros_like(p)
+    for s in _PERMS:
+        out += np.transpose(p, s + tuple(3 + x for x in s))
+    return out
+
+
+def w_dense(t2: np.ndarray, vm: np.ndarray, vvvo: np.ndarray) -> np.ndarray:
+    """W[i,j,k,a,b,c] for every ordered occupied triple."""
+    part = np.einsum("dbai,cdjk->ijkabc", vvvo, t2, optimize=True)
+    hole = np.einsum("abli,lckj->ijkabc", t2, vm, optimize=True)
+    return _sym6(part - hole)
+
+
+def _vp(w: np.ndarray, s: str) -> np.ndarray:
+    """Virtual-index permutation: result[...,a,b,c] = w[..., s] with s a word over abc."""
+    return np.einsum("ijk" + s + "->ijkabc", w)
+
+
+def denominators(eh: np.ndarray, ep: np.ndarray) -> np.ndarray:
+    o = eh[:, None, None] + eh[None, :, None] + eh[None, None, :]
+    v = ep[:, None, None] + ep[None, :, None] + ep[None, None, :]
+    return o[:, :, :, None, None, None] - v[None, None, None, :, :, :]
+
+
+def x_tuples(w: np.ndarray, d: np.ndarray) -> np.ndarray:
+    """X_ijk = sum_abc W (8W - 4(W_acb+W_cba+W_bac) + 2(W_bca+W_cab)) / D, a=b=c excluded."""
+    f = 8.0 * w - 4.0 * (_vp(w, "acb") + _vp(w, "cba") + _vp(w, "bac")) \
+        + 2.0 * (_vp(w, "bca") + _vp(w, "cab"))
+    term = w * f / d
+    nu = w.shape[3]
+    a = np.arange(nu)
+    term[:, :, :, a, a, a] = 0.0
+    return term.sum(axis=(3, 4, 5))
+
+
+def z_dense(w: np.ndarray, d: np.ndarray) -> np.ndarray:
+    """Z_pqr(a,b,c) = (2W_abc - 2W_bac - W_acb + W_bca) / D for every ordered triple."""
+    return (2.0 * w - 2.0 * _vp(w, "bac") - _vp(w, "acb") + _vp(w, "bca")) / d
+
+
+def tuple_weights(no: int) -> np.ndarray:
+    """Weights of the driver's tuple set: 1 for i>j>k, 1/2 for ijj and iij (i>j)."""
+    wt = np.zeros((no, no, no))
+    for i in range(no):
+        for j in range(i):
+            wt[i, j, j] = 0.5
+            wt[i, i, j] = 0.5
+            for k in range(j):
+                wt[i, j, k] = 1.0
+    return wt
+
+
+def energies(inp: dict) -> dict:
+    """etd, ets, E(T) and v1(nu,no) from the dense closed forms."""
+    w = w_dense(inp["t2"], inp["vm"], inp["vvvo"])
+    d = denominators(inp["eh"], inp["ep"])
+    x = x_tuples(w, d)
+    etd = float((tuple_weights(inp["no"]) * x).sum())
+    etd_sixth = float(x.sum() / 6.0)
+    z = z_dense(w, d)
+    v1 = np.einsum("pqrabc,bcqr->ap", z, inp["voe"], optimize=True)
+    ets = float(2.0 * np.einsum("pa,ap->", inp["t1"], v1))
+    return {"etd": etd, "etd_sixth": etd_sixth, "ets": ets, "e": etd + ets,
+            "v1": v1, "w": w, "x": x, "z": z}
+
+
+def grad_voe(inp: dict, z: np.ndarray | None = None) -> np.ndarray:
+    """dE/dVOE(b,c,q,r) = 2 sum_{p,a} T1(p,a) Z_pqr(a,b,c) (T2 held fixed)."""
+    if z is None:
+        w = w_dense(inp["t2"], inp["vm"], inp["vvvo"])
+        z = z_dense(w, denominators(inp["eh"], inp["ep"]))
+    return 2.0 * np.einsum("pa,pqrabc->bcqr", inp["t1"], z, optimize=True)
+
+
+def textbook(no: int, nu: int, eh, ep, t1, eri) -> dict:
+    """Closed-shell (T) from (pq|rs), t1 and MP2-like t2, in physics index conventions."""
+    o = slice(0, no)
+    v = slice(no, no + nu)
+    ovov = eri[o, v, o, v]                                    # (ia|jb)
+    dmp2 = eh[:, None, None, None] + eh[None, None, :, None] \
+        - ep[None, :, None, None] - ep[None, None, None, :]
+    t2 = np.einsum("iajb->ijab", ovov / dmp2)                 # t_ij^ab = (ia|jb)/D_ij^ab
+    part = np.einsum("bdai,kjcd->ijkabc", eri[v, v, v, o], t2, optimize=True)
+    hole = np.einsum("ckjl,ilab->ijkabc", eri[v, o, o, o], t2, optimize=True)
+    w = _sym6(part - hole)
+    y = (np.einsum("ia,jbkc->ijkabc", t1, ovov, optimize=True)
+         + np.einsum("jb,iakc->ijkabc", t1, ovov, optimize=True)
+         + np.einsum("kc,iajb->ijkabc", t1, ovov, optimize=True))
+    d = denominators(eh, ep)
+    left = 4.0 * w + _vp(w, "bca") + _vp(w, "cab")
+    e4 = (left * (w - _vp(w, "cba")) / (3.0 * d)).sum()
+    est = (left * (y - _vp(y, "cba")) / (3.0 * d)).sum()
+    return {"etd": float(e4), "ets": float(est), "e": float(e4 + est)}

@@ -1,0 +1,195 @@
+"""Parity of the HIP path (through the C ABI) with the reference's own results.
+
+Golden vectors come from the reference's compiled per-tuple routines
+(tests/golden/golden.npz); the C restatement (oracle/port.c) covers cases the golden
+file does not hold.  Bar: |dE| <= 1e-9 Eh and 1e-12 relative, v1 to 1e-10 relative."""
+
+import ctypes
+
+import numpy as np
+import pytest
+from conftest import GOLDEN_CASES
+
+from libcc_amd import lib, synth
+from libcc_amd.triples import TriplesContext, cc_triples
+from oracle import dense, port
+
+pytestmark = pytest.mark.gpu
+
+
+def _energy_ok(got_ets, got_etd, ets, etd):
+    scale = max(1.0, abs(etd))
+    assert abs(got_etd - etd) <= max(1e-9, 1e-12 * scale), (got_etd, etd)
+    assert abs(got_ets - ets) <= max(1e-9, 1e-12 * scale), (got_ets, ets)
+
+
+def _v1_ok(got, want):
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * max(1e-300, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES, ids=lambda c: "no%d_nu%d_s%d" % c)
+def test_energy_matches_reference_golden(golden, case):
+    inp = synth.make_inputs(*case)
+    tag = "%d_%d_%d" % case
+    ets, etd, v1 = cc_triples(inp["eh"], inp["ep"], inp["t1"], inp["t2"], inp["vm"], inp["voe"], inp["vvvo"])
+    _energy_ok(ets, etd, *golden["e_" + tag])
+    _v1_ok(v1, golden["v1_" + tag])
+
+
+@pytest.mark.parametrize("case", [(3, 5, 7), (4, 36, 1001)], ids=lambda c: "no%d_nu%d" % c[:2])
+def test_packed_vvvo_input(golden, case):
+    inp = synth.make_inputs(*case)
+    tag = "%d_%d_%d" % case
+    ets, etd, v1 = cc_triples(inp["eh"], inp["ep"], inp["t1"], inp["t2"], inp["vm"], inp["voe"],
+                              synth.pack_vvvo(inp["vvvo"]), packed_vvvo=True)
+    _energy_ok(ets, etd, *golden["e_" + tag])
+    _v1_ok(v1, golden["v1_" + tag])
+
+
+@pytest.mark.parametrize("no,nu,seed", [(1, 5, 4), (2, 5, 4), (2, 16, 3), (3, 17, 6), (5, 1, 2), (3, 33, 8),
+                                        (7, 20, 12)])
+def test_edge_shapes_match_port(no, nu, seed):
+    """No tuples (no=1), degenerate tuples only (no=2), nu at / just past a 16 boundary, nu=1."""
+    inp = synth.make_inputs(no, nu, seed)
+    want = port.triples(inp)
+    ets, etd, v1 = cc_triples(inp["eh"], inp["ep"], inp["t1"], inp["t2"], inp["vm"], inp["voe"], inp["vvvo"])
+    _energy_ok(ets, etd, want["ets"], want["etd"])
+    _v1_ok(v1, want["v1"])
+    if no == 1:
+        assert etd == 0.0 and ets == 0.0
+
+
+def test_tuple_w_matches_reference_cube(golden):
+    inp = synth.make_inputs(4, 6, 11)
+    ctx = TriplesContext(4, 6)
+    ctx.load(inp)
+    for (i, j, k) in [(2, 1, 0), (3, 2, 1), (3, 1, 0)]:
+        w = ctx.tuple_w(i + 1, j + 1, k + 1)
+        a = np.arange(6)
+        w[a, a, a] = 0.0
+        np.testing.assert_allclose(w, golden[f"w_0_{i}_{j}_{k}"], rtol=0, atol=1e-16)
+    # degenerate tuples against the closed form
+    wd = dense.w_dense(inp["t2"], inp["vm"], inp["vvvo"])
+    for (i, j, k) in [(2, 1, 1), (3, 3, 0)]:
+        np.testing.assert_allclose(ctx.tuple_w(i + 1, j + 1, k + 1), wd[i, j, k], rtol=0, atol=1e-16)
+    ctx.close()
+
+
+def test_run_to_run_bitwise_deterministic():
+    inp = synth.make_inputs(6, 40, 77)
+    ctx = TriplesContext(6, 40)
+    ctx.load(inp)
+    r1 = ctx.energy()
+    r2 = ctx.energy()
+    assert r1["etd"] == r2["etd"] and r1["ets"] == r2["ets"]
+    assert np.array_equal(r1["v1"], r2["v1"])
+    ctx.close()
+
+
+def test_scaling_property_at_adjoint_size(golden):
+    """C3 shape (no=8, nu=120): W is linear in (VVVO, VM) at fixed T2."""
+    inp = synth.make_inputs(8, 120, 1003)
+    ets0, etd0 = golden["e_8_120_1003"]
+    s = 0.5
+    sc = dict(inp, vvvo=np.asfortranarray(inp["vvvo"] * s), vm=np.asfortranarray(inp["vm"] * s))
+    ets, etd, _ = cc_triples(*[sc[k] for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")])
+    assert abs(etd - s * s * etd0) <= 1e-12 * abs(etd0)
+    assert abs(ets - s * ets0) <= 1e-11 * abs(etd0)
+
+
+# ---------------------------------------------------------------- legacy Fortran ABI
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i64(v):
+    return ctypes.byref(ctypes.c_int64(v))
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_legacy_cc_triples_acc(golden, layout):
+    no, nu, seed = 4, 36, 1001
+    inp = synth.make_inputs(no, nu, seed)
+    L = lib()
+    L.common_cc_init_(_i64(no), _i64(nu))
+    vv = inp["vvvo"] if layout == 0 else synth.pack_vvvo(inp["vvvo"])
+    vv = np.asfortranarray(vv)
+    L.ccx_register_vvvo_(_i64(4242), _dp(vv), _i64(layout))
+    arrs = {k: np.asfortranarray(inp[k].copy()) for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    v1 = np.zeros((nu, no), order="F")
+    v3 = np.zeros(nu ** 3)
+    t3 = np.zeros(nu ** 3)
+    e_s, e_d = ctypes.c_double(), ctypes.c_double()
+    L.cc_triples_acc_(_dp(arrs["eh"]), _dp(arrs["ep"]), _dp(v1), _dp(arrs["t1"]), _dp(arrs["t2"]), _dp(v3),
+                      _dp(t3), _dp(arrs["vm"]), _dp(arrs["voe"]), _i64(4242), ctypes.byref(e_s), ctypes.byref(e_d))
+    assert L.ccx_last_status() == 0
+    ets, etd = golden["e_4_36_1001"]
+    _energy_ok(e_s.value, e_d.value, ets, etd)
+    _v1_ok(v1, golden["v1_4_36_1001"])
+    assert L.ccx_get_ets_() == e_s.value and L.ccx_get_etd_() == e_d.value
+    # t1 is transposed in place to (nu,no), as the reference's TRPOSE leaves it
+    np.testing.assert_array_equal(arrs["t1"].reshape(nu, no, order="F"), inp["t1"].T)
+
+
+def test_legacy_cc_triples_via_d_vvvo(golden):
+    no, nu, seed = 3, 5, 7
+    inp = synth.make_inputs(no, nu, seed)
+    L = lib()
+    L.common_cc_init_(_i64(no), _i64(nu))
+    vv = np.asfortranarray(inp["vvvo"])
+    L.ccx_register_vvvo_(_i64(7), _dp(vv), _i64(0))
+    L.ccx_set_d_vvvo_(_i64(7))
+    arrs = {k: np.asfortranarray(inp[k].copy()) for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    v1 = np.zeros((nu, no), order="F")
+    scratch = [np.zeros(nu ** 3) for _ in range(5)]
+    L.cc_triples_(_dp(arrs["eh"]), _dp(arrs["ep"]), _dp(v1), _dp(arrs["t1"]), _dp(arrs["t2"]), _dp(scratch[0]),
+                  _dp(scratch[1]), _dp(arrs["vm"]), _dp(arrs["voe"]), _dp(scratch[2]), _dp(scratch[3]),
+                  _dp(scratch[4]))
+    assert L.ccx_last_status() == 0
+    _energy_ok(L.ccx_get_ets_(), L.ccx_get_etd_(), *golden["e_3_5_7"])
+    _v1_ok(v1, golden["v1_3_5_7"])
+
+
+def test_legacy_ijk_tuple(golden):
+    no, nu = 4, 6
+    inp = synth.make_inputs(no, nu, 11)
+    L = lib()
+    L.common_cc_init_(_i64(no), _i64(nu))
+    t2, vm, vvvo = inp["t2"], inp["vm"], inp["vvvo"]
+    i, j, k = 3, 2, 1                                   # 0-based (4,3,2) in Fortran terms
+    f = np.asfortranarray
+    t2x = {x: f(t2[:, :, :, x]) for x in (i, j, k)}
+    vmxy = {(x, y): f(vm[:, :, x, y]) for x in (i, j, k) for y in (i, j, k) if x != y}
+    ve = {x: f(np.einsum("dqp->dpq", vvvo[:, :, :, x])) for x in (i, j, k)}   # VE_x(d,p,q)=VVVO(d,q,p,x)
+    v3 = np.zeros((nu, nu, nu), order="F")
+    L.ijk_tuple_(_i64(i + 1), _i64(j + 1), _i64(k + 1), _dp(t2x[i]), _dp(t2x[j]), _dp(t2x[k]),
+                 _dp(vmxy[(i, j)]), _dp(vmxy[(j, i)]), _dp(vmxy[(i, k)]), _dp(vmxy[(k, i)]), _dp(vmxy[(j, k)]),
+                 _dp(vmxy[(k, j)]), _dp(ve[i]), _dp(ve[j]), _dp(ve[k]), _dp(v3))
+    assert L.ccx_last_status() == 0
+    np.testing.assert_allclose(v3, golden[f"w_0_{i}_{j}_{k}"], rtol=0, atol=1e-16)
+
+
+def test_device_resident_inputs(golden):
+    torch = pytest.importorskip("torch")
+    inp = synth.make_inputs(4, 87, 1002)
+    dev = {}
+    for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo"):
+        a = np.ravel(inp[k], order="F")
+        dev[k] = torch.from_numpy(a.copy()).to("cuda:0")
+    ctx = TriplesContext(4, 87)
+    ctx.load(dev)
+    v1d = torch.zeros(87 * 4, dtype=torch.float64, device="cuda:0")
+    r = ctx.energy(v1_device=v1d)
+    _energy_ok(r["ets"], r["etd"], *golden["e_4_87_1002"])
+    _v1_ok(v1d.cpu().numpy().reshape(87, 4, order="F"), golden["v1_4_87_1002"])
+    ctx.close()
+
+
+def test_nonfinite_input_rejected():
+    inp = synth.make_inputs(3, 5, 7)
+    bad = dict(inp, t2=inp["t2"].copy())
+    bad["t2"][1, 2, 0, 1] = np.nan
+    with pytest.raises(Exception) as e:
+        cc_triples(*[bad[k] for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")])
+    assert "status 5" in str(e.value)

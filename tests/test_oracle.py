@@ -1,0 +1,172 @@
+"""Pins the checkers: the C restatement (oracle/port.c), the dense closed forms and the
+textbook formula against the reference's own compiled routines (golden vectors) and
+against each other, plus invariants that a plausible mistake would break."""
+
+import numpy as np
+import pytest
+from conftest import GOLDEN_CASES, SMALL_CASES
+
+from libcc_amd import synth
+from oracle import dense, port
+
+from golden.make_golden import TUPLES, checksum
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES[:7])
+def test_generator_matches_golden_inputs(golden, case):
+    inp = synth.make_inputs(*case)
+    tag = "%d_%d_%d" % case
+    np.testing.assert_allclose(checksum(inp), golden["sum_" + tag], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES[:7])
+def test_port_matches_reference_golden(golden, case):
+    inp = synth.make_inputs(*case)
+    tag = "%d_%d_%d" % case
+    r = port.triples(inp)
+    ets, etd = golden["e_" + tag]
+    scale = max(1.0, abs(etd))
+    assert abs(r["etd"] - etd) <= 1e-12 * scale
+    assert abs(r["ets"] - ets) <= 1e-12 * scale
+    np.testing.assert_allclose(r["v1"], golden["v1_" + tag], rtol=1e-10, atol=1e-13 * np.abs(golden["v1_" + tag]).max())
+
+
+@pytest.mark.parametrize("kind,i,j,k", TUPLES)
+def test_port_tuple_matches_reference_golden(golden, kind, i, j, k):
+    inp = synth.make_inputs(4, 6, 11)
+    tag = f"{kind}_{i}_{j}_{k}"
+    r = port.tuple_(inp, kind, i, j, k)
+    np.testing.assert_allclose(r["v3"], golden["w_" + tag], rtol=0, atol=1e-16)
+    np.testing.assert_allclose(r["v1"], golden["tv1_" + tag], rtol=0, atol=1e-17)
+    assert abs(r["etd"] - golden["tetd_" + tag][0]) < 1e-18
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_cube_operations_match_reference_tables(golden, n):
+    """trant3_1/3/4/5, TRANT3_ACC 1..5, tranmd_23, SYMT311 23/12/13 on the 1..n^3 cube."""
+    ops = [1, 3, 4, 5, 1, 2, 3, 4, 5, 1, 23, 12, 13]
+    tabs = golden[f"cube_{n}"]
+    for idx, op in enumerate(ops):
+        c = np.arange(1, n ** 3 + 1, dtype=float).reshape(n, n, n, order="F")
+        out = port.cube_op(c, op)
+        assert np.array_equal(out.ravel(order="F"), tabs[idx]), (idx, op)
+
+
+def test_cube_permutation_closed_forms():
+    """new(x,y,z) = old(s(x,y,z)) for the five ids, on a cube of distinct values."""
+    rng = np.random.default_rng(0)
+    c = rng.permutation(5 ** 3).astype(float).reshape(5, 5, 5, order="F")
+    sig = {1: "xzy", 2: "yxz", 3: "zyx", 4: "yzx", 5: "zxy"}
+    for op, s in sig.items():
+        want = np.einsum(s + "->xyz", c)
+        assert np.array_equal(port.cube_op(c, op), want), op
+
+
+@pytest.mark.parametrize("case", SMALL_CASES)
+def test_dense_closed_form_matches_port(case):
+    inp = synth.make_inputs(*case)
+    r = dense.energies(inp)
+    p = port.triples(inp)
+    assert abs(r["etd"] - p["etd"]) < 1e-15
+    assert abs(r["etd_sixth"] - p["etd"]) < 1e-15
+    assert abs(r["ets"] - p["ets"]) < 1e-17
+    np.testing.assert_allclose(r["v1"], p["v1"], rtol=0, atol=1e-16)
+
+
+@pytest.mark.parametrize("case", SMALL_CASES)
+def test_textbook_formula_matches(case):
+    """Independent closed-shell (T) from raw (pq|rs): pins layouts, index roles and weights."""
+    no, nu, seed = case
+    inp = synth.make_inputs(no, nu, seed)
+    eh, ep, t1, eri = synth.full_eri(no, nu, seed)
+    tb = dense.textbook(no, nu, eh, ep, t1, eri)
+    p = port.triples(inp)
+    assert abs(tb["etd"] - p["etd"]) < 1e-15
+    assert abs(tb["ets"] - p["ets"]) < 1e-17
+
+
+def test_dense_w_matches_reference_cube(golden):
+    inp = synth.make_inputs(4, 6, 11)
+    w = dense.w_dense(inp["t2"], inp["vm"], inp["vvvo"])
+    for kind, i, j, k in TUPLES:
+        if kind != 0:
+            continue
+        ref = golden[f"w_{kind}_{i}_{j}_{k}"]
+        got = w[i, j, k].copy()
+        a = np.arange(6)
+        got[a, a, a] = 0.0
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-16)
+
+
+def test_w_pair_permutation_symmetry():
+    inp = synth.make_inputs(4, 5, 3)
+    w = dense.w_dense(inp["t2"], inp["vm"], inp["vvvo"])
+    # W_{jik}(b,a,c) = W_ijk(a,b,c), W_{kij}(c,a,b) = W_ijk(a,b,c)
+    np.testing.assert_allclose(np.einsum("jikbac->ijkabc", w), w, rtol=0, atol=1e-17)
+    np.testing.assert_allclose(np.einsum("kijcab->ijkabc", w), w, rtol=0, atol=1e-17)
+
+
+def test_invariants():
+    inp = synth.make_inputs(4, 6, 5)
+    base = port.triples(inp)
+    # T1 = 0  =>  ets = 0 ; VOE = 0 => ets = 0 (T2 held fixed)
+    z = dict(inp, t1=np.zeros_like(inp["t1"]))
+    assert port.triples(z)["ets"] == 0.0
+    z = dict(inp, voe=np.zeros_like(inp["voe"]))
+    assert port.triples(z)["ets"] == 0.0
+    # W is linear in (VVVO, VM) at fixed T2: etd scales by s^2, ets by s
+    s = 1.7
+    sc = dict(inp, vvvo=np.asfortranarray(inp["vvvo"] * s), vm=np.asfortranarray(inp["vm"] * s))
+    r = port.triples(sc)
+    assert abs(r["etd"] - s * s * base["etd"]) < 1e-14 * abs(base["etd"]) * 10
+    assert abs(r["ets"] - s * base["ets"]) < 1e-12 * abs(base["ets"]) * 10
+    # ets is linear in T1
+    t = dict(inp, t1=np.asfortranarray(inp["t1"] * -3.0))
+    assert abs(port.triples(t)["ets"] + 3.0 * base["ets"]) < 1e-13 * abs(base["ets"]) * 10
+
+
+def test_relabelling_invariance():
+    """E is invariant under a relabelling of occupied (and of virtual) orbitals applied to
+    every array and orbital energy consistently."""
+    inp = synth.make_inputs(4, 6, 9)
+    base = port.triples(inp)
+    po = np.array([2, 0, 3, 1])
+    pv = np.array([5, 1, 0, 4, 2, 3])
+    f = np.asfortranarray
+    rl = dict(inp,
+              eh=f(inp["eh"][po]), ep=f(inp["ep"][pv]), t1=f(inp["t1"][np.ix_(po, pv)]),
+              t2=f(inp["t2"][np.ix_(pv, pv, po, po)]), vm=f(inp["vm"][np.ix_(po, pv, po, po)]),
+              voe=f(inp["voe"][np.ix_(pv, pv, po, po)]), vvvo=f(inp["vvvo"][np.ix_(pv, pv, pv, po)]))
+    r = port.triples(rl)
+    assert abs(r["etd"] - base["etd"]) < 1e-15
+    assert abs(r["ets"] - base["ets"]) < 1e-17
+    np.testing.assert_allclose(r["v1"], base["v1"][np.ix_(pv, po)], rtol=0, atol=1e-16)
+
+
+def test_degenerate_only_and_no_tuples():
+    # no=1: no tuple at all; no=2: only (1,0,0) and (1,1,0)
+    for no in (1, 2):
+        inp = synth.make_inputs(no, 5, 4)
+        r = port.triples(inp)
+        d = dense.energies(inp)
+        assert abs(r["etd"] - d["etd"]) < 1e-16 and abs(r["ets"] - d["ets"]) < 1e-18
+    assert port.triples(synth.make_inputs(1, 5, 4))["etd"] == 0.0
+
+
+def test_packed_vvvo_layout():
+    inp = synth.make_inputs(3, 5, 7)
+    pk = synth.pack_vvvo(inp["vvvo"])
+    nu, no = 5, 3
+    assert pk.shape == (nu * (nu + 1) // 2, nu * no)
+    # row p*(p-1)/2+q (1-based, q<=p) of column c + nu*(x-1) holds VVVO(p,q,c,x)
+    for (p, q, c, x) in [(1, 1, 1, 1), (5, 2, 3, 2), (4, 4, 5, 3), (3, 1, 2, 1)]:
+        row = p * (p - 1) // 2 + q
+        col = c + nu * (x - 1)
+        assert pk[row - 1, col - 1] == inp["vvvo"][p - 1, q - 1, c - 1, x - 1]
+        assert pk[row - 1, col - 1] == inp["vvvo"][q - 1, p - 1, c - 1, x - 1]
+
+
+def test_task_decode_order():
+    no = 6
+    want = [(i, j, k) for i in range(no) for j in range(i) for k in range(j)]
+    assert [port.task(t, no) for t in range(len(want))] == want

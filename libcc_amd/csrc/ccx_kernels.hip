@@ -26,7 +26,7 @@
 // of which LDS-staged operand panel a fragment is read from and in which orientation.
 //
 // FP64 MFMA: v_mfma_f64_16x16x4_f64.  Per 16x16 output tile, lane l supplies
-// A[l%16][l/16], B[l/16][l%16] and holds D[4*(l/16)+v][l%16], v=0..3.
+// A[l%16][l/16], B[l/16][l%16] and holds D[l/16 + 4v][l%16], v=0..3 (checked on the device).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <type_traits>
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(NTHR, 1) triples_f64_kernel(TriplesArgs g)
                 for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int a = 4 * (lane >> 4) + v, b = lane & 15, c = wave * 4 + bb;
+                        const int a = (lane >> 4) + 4 * v, b = lane & 15, c = wave * 4 + bb;
                         lds[pp * TCUBE + a * TSTR_A + b * TSTR_B + c] = acc[half * 3 + pp][bb][v];
                     }
             __syncthreads();
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(NTHR, 1) triples_f64_kernel(TriplesArgs g)
                 for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int a = 4 * (lane >> 4) + v, b = wave * 4 + bb, c = lane & 15;
+                        const int a = (lane >> 4) + 4 * v, b = wave * 4 + bb, c = lane & 15;
                         acc[half * 3 + pp][bb][v] = lds[pp * TCUBE + a * TSTR_A + b * TSTR_B + c];
                     }
             __syncthreads();
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(NTHR, 1) triples_f64_kernel(TriplesArgs g)
             const int bg = B * TB + bl;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int al = 4 * (lane >> 4) + v;
+                const int al = (lane >> 4) + 4 * v;
                 const int ag = A * TB + al;
                 if (ag >= g.nu || bg >= g.nu || cg >= g.nu) continue;
                 double w[6];
@@ -467,13 +467,13 @@ __global__ void __launch_bounds__(NTHR, 1) triples_f64_kernel(TriplesArgs g)
 
         // ---- deterministic reductions
         double *rw = red[wave];
-        // role a: rows 4*(lane>>4)+v, reduce over the 16 lanes of a group, then waves
+        // role a: rows (lane>>4)+4v, reduce over the 16 lanes of a group, then waves
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
                 const double s = group16_sum(va[v][p]);
-                if ((lane & 15) == 0) rw[(4 * (lane >> 4) + v) * 3 + p] = s;
+                if ((lane & 15) == 0) rw[((lane >> 4) + 4 * v) * 3 + p] = s;
             }
         // role b: rows wave*4+bb, whole-wave sums
 #pragma unroll

@@ -4,6 +4,14 @@ import sys
 import numpy as np
 import pytest
 
+# torch (used by a few GPU tests for device-resident inputs) bundles its own HIP runtime and
+# links it by the unversioned name; importing it before libcc_mi355x.so is loaded makes the
+# library bind to that same runtime (matching soname) instead of a second copy.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(__file__))

@@ -24,7 +24,7 @@ def _energy_ok(got_ets, got_etd, ets, etd):
 
 
 def _v1_ok(got, want):
-    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * max(1e-300, np.abs(want).max()))
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=max(1e-12 * np.abs(want).max(), 1e-30))
 
 
 @pytest.mark.parametrize("case", GOLDEN_CASES, ids=lambda c: "no%d_nu%d_s%d" % c)

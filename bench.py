@@ -76,7 +76,19 @@ def main() -> None:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path's tuples would shard over GPUs with one final "
               f"reduction (see DESIGN.md), but no multi-GPU run is part of this build (--gpus {args.gpus}).",
@@ -94,6 +106,7 @@ def main() -> None:
     torch.cuda.synchronize()
     ctx = TriplesContext(no, nu)
     log("warmup")
+    ref = None
     for _ in range(args.warmup):
         ctx.load(inp)
         ref = ctx.energy("fp64")
@@ -108,7 +121,7 @@ def main() -> None:
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
-    if r["etd"] != ref["etd"] or r["ets"] != ref["ets"]:
+    if ref is not None and (r["etd"] != ref["etd"] or r["ets"] != ref["ets"]):
         raise RuntimeError("non-deterministic energy between steps")
     kernel_s = sum(kms) / len(kms) / 1e3
     flops = algorithmic_flops(no, nu)

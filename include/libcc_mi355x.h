@@ -39,8 +39,10 @@ extern "C" {
 #define CCX_ERR_INPUT 5    /* NaN/Inf in an input array                       */
 #define CCX_ERR_FORM 6     /* arithmetic form not available                   */
 
-/* arithmetic forms of the triples contraction */
+/* arithmetic forms of the triples contraction; denominators, energies, v1 and all
+ * reductions are FP64 in every form */
 #define CCX_FORM_FP64 0    /* full FP64 (v_mfma_f64_16x16x4_f64)              */
+#define CCX_FORM_FP32 1    /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32) */
 
 const char *ccx_error_string(int status);
 int ccx_last_status(void);          /* status of the last legacy (void) call    */

@@ -18,12 +18,13 @@ const char *ccx_hip_last_error(void);
 int ccx_hip_lds_bytes(void);
 
 int ccx_hip_unpack_vvvo(const double *packed, double *full, int no, int nu, void *stream);
-int ccx_hip_pack(const double *vvvo, const double *t2, const double *vm, const double *voe, double *ax,
-                 double *bx, double *voep, int no, int nu, int nup, int kp, void *stream);
-int ccx_hip_triples_f64(const double *ax, const double *bx, const double *voep, const double *eh,
-                        const double *ep, const int *tuples, const int *blocks, double *e_part, double *v1_part,
-                        double *wout, int no, int nu, int nup, int kp, int ntup, int nblk, int grid,
-                        void *stream);
+int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *vm, const double *voe, void *axr,
+                 void *axc, void *bx, double *voep, int no, int nu, int nup, int kp, void *stream);
+int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
+                    const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
+                    double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
+                    int ntup, int nblk, int grid, void *stream);
+int ccx_hip_to_f32(const double *src, float *dst, long long n, void *stream);
 int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tuples, const int *blocks,
                      const double *t1, double *v1t, double *v1, double *out, int no, int nu, int ntup, int nblk,
                      void *stream);

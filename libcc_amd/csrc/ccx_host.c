@@ -18,8 +18,11 @@ struct ccx_ctx {
     int64_t ntup, nblk;
     void *stream;
     int loaded;
-    double *d_ax, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
+    double *d_axr, *d_axc, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
+    float *d_axr32, *d_axc32, *d_bx32; /* FP32 form slabs, converted on first use after a load */
+    int f32_ready;
     int *d_tuples, *d_blocks;
+    unsigned int *d_counter;
     double *d_epart, *d_v1part, *d_v1t, *d_v1, *d_out;
     int *h_tuples, *h_blocks;
     double kernel_ms, total_ms;
@@ -45,8 +48,8 @@ const char *ccx_error_string(int s)
 /* Occupied tuples in the reference driver's order: distinct i>j>k with i outermost
  * (DDCC_T_TASK, src/triples-ijk/common_functions.f:62-86), then for each i>j the
  * (i,j,j) and (i,i,j) tuples (src/triples-ijk/triples.F90:237-300).  0-based; the 4th
- * entry is 1 for a distinct tuple and 2 for a degenerate one (energy weight 1/2,
- * src/triples-ijk/t1wt3.f90:188-192). */
+ * entry is 1 for a distinct tuple, 2 for (i,j,j) and 3 for (i,i,j) (degenerate tuples
+ * carry energy weight 1/2, src/triples-ijk/t1wt3.f90:188-192). */
 static int64_t build_tuples(int64_t no, int *out)
 {
     int64_t n = 0;
@@ -60,7 +63,7 @@ static int64_t build_tuples(int64_t no, int *out)
         for (int64_t j = 0; j < i; ++j) {
             if (out) {
                 out[4 * n] = (int)i; out[4 * n + 1] = (int)j; out[4 * n + 2] = (int)j; out[4 * n + 3] = 2;
-                out[4 * n + 4] = (int)i; out[4 * n + 5] = (int)i; out[4 * n + 6] = (int)j; out[4 * n + 7] = 2;
+                out[4 * n + 4] = (int)i; out[4 * n + 5] = (int)i; out[4 * n + 6] = (int)j; out[4 * n + 7] = 3;
             }
             n += 2;
         }
@@ -88,7 +91,7 @@ static int64_t build_blocks(int64_t nb, int *out)
 
 static void free_dev(ccx_ctx *c)
 {
-    double **d[] = {&c->d_ax, &c->d_bx, &c->d_voep, &c->d_eh, &c->d_ep, &c->d_t1,
+    double **d[] = {&c->d_axr, &c->d_axc, &c->d_bx, &c->d_voep, &c->d_eh, &c->d_ep, &c->d_t1,
                     &c->d_epart, &c->d_v1part, &c->d_v1t, &c->d_v1, &c->d_out};
     for (size_t q = 0; q < sizeof(d) / sizeof(d[0]); ++q) {
         ccx_hip_free(*d[q]);
@@ -96,7 +99,14 @@ static void free_dev(ccx_ctx *c)
     }
     ccx_hip_free(c->d_tuples);
     ccx_hip_free(c->d_blocks);
+    ccx_hip_free(c->d_counter);
+    ccx_hip_free(c->d_axr32);
+    ccx_hip_free(c->d_axc32);
+    ccx_hip_free(c->d_bx32);
     c->d_tuples = c->d_blocks = NULL;
+    c->d_counter = NULL;
+    c->d_axr32 = c->d_axc32 = c->d_bx32 = NULL;
+    c->f32_ready = 0;
     c->loaded = 0;
 }
 
@@ -169,8 +179,10 @@ static int build_from_device(ccx_ctx *c, const double *d_eh, const double *d_ep_
     const int64_t no = c->no, nu = c->nu, nup = c->nup, nb = c->nb, kp = c->kp;
     const int64_t nitems = c->ntup * c->nblk;
     double *h_ep = NULL;
-    if (c->d_ax) goto allocated; /* buffers depend only on the dimensions: reuse */
-    DALLOC(c->d_ax, no * nb * nb * kp * 256);
+    if (c->d_axr) goto allocated; /* buffers depend only on the dimensions: reuse */
+    DALLOC(c->d_axr, no * nb * nb * kp * 256);
+    DALLOC(c->d_axc, no * nb * nb * kp * 256);
+    DALLOC(c->d_counter, 4);
     DALLOC(c->d_bx, no * no * kp * nup);
     DALLOC(c->d_voep, no * no * nup * nup);
     DALLOC(c->d_eh, no);
@@ -185,8 +197,9 @@ static int build_from_device(ccx_ctx *c, const double *d_eh, const double *d_ep_
     DALLOC(c->d_out, 4);
 allocated:
     c->loaded = 0;
-    if (ccx_hip_pack(d_vvvo_full, d_t2, d_vm, d_voe, c->d_ax, c->d_bx, c->d_voep, (int)no, (int)nu, (int)nup,
-                     (int)kp, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    c->f32_ready = 0;
+    if (ccx_hip_pack(0, d_vvvo_full, d_t2, d_vm, d_voe, c->d_axr, c->d_axc, c->d_bx, c->d_voep, (int)no, (int)nu,
+                     (int)nup, (int)kp, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
     /* eps_virtual padded to nup with finite values (padded elements carry W = 0) */
     h_ep = (double *)malloc(sizeof(double) * (size_t)nup);
     if (!h_ep) { rc = CCX_ERR_ALLOC; goto fail; }
@@ -271,19 +284,48 @@ static int grid_size(const ccx_ctx *c, int64_t nitems)
     (void)c;
 }
 
+/* FP32 slabs from the FP64 ones (same layout), allocated on first use */
+static int ensure_f32(ccx_ctx *c)
+{
+    if (c->f32_ready) return CCX_OK;
+    const int64_t nax = c->no * c->nb * c->nb * c->kp * 256, nbx = c->no * c->no * c->kp * c->nup;
+    if (!c->d_axr32) {
+        if (ccx_hip_malloc((void **)&c->d_axr32, sizeof(float) * (size_t)nax) != 0 ||
+            ccx_hip_malloc((void **)&c->d_axc32, sizeof(float) * (size_t)nax) != 0 ||
+            ccx_hip_malloc((void **)&c->d_bx32, sizeof(float) * (size_t)nbx) != 0) {
+            ccx_hip_free(c->d_axr32); ccx_hip_free(c->d_axc32); ccx_hip_free(c->d_bx32);
+            c->d_axr32 = c->d_axc32 = c->d_bx32 = NULL;
+            return CCX_ERR_ALLOC;
+        }
+    }
+    if (ccx_hip_to_f32(c->d_axr, c->d_axr32, nax, c->stream) != 0 ||
+        ccx_hip_to_f32(c->d_axc, c->d_axc32, nax, c->stream) != 0 ||
+        ccx_hip_to_f32(c->d_bx, c->d_bx32, nbx, c->stream) != 0)
+        return CCX_ERR_DEVICE;
+    c->f32_ready = 1;
+    return CCX_OK;
+}
+
 int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v1_on_device)
 {
     if (!c) return CCX_ERR_ARG;
     if (!c->loaded) return CCX_ERR_STATE;
-    if (form != CCX_FORM_FP64) return CCX_ERR_FORM;
+    if (form != CCX_FORM_FP64 && form != CCX_FORM_FP32) return CCX_ERR_FORM;
     const int64_t nitems = c->ntup * c->nblk;
     double host_out[2] = {0.0, 0.0};
+    if (form == CCX_FORM_FP32) {
+        int rc = ensure_f32(c);
+        if (rc != CCX_OK) return rc;
+    }
+    const void *axr = form == CCX_FORM_FP32 ? (const void *)c->d_axr32 : (const void *)c->d_axr;
+    const void *axc = form == CCX_FORM_FP32 ? (const void *)c->d_axc32 : (const void *)c->d_axc;
+    const void *bx = form == CCX_FORM_FP32 ? (const void *)c->d_bx32 : (const void *)c->d_bx;
     if (ccx_hip_event_record(c->ev[0], c->stream) != 0) return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[1], c->stream) != 0) return CCX_ERR_DEVICE;
     if (nitems > 0 &&
-        ccx_hip_triples_f64(c->d_ax, c->d_bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
-                            c->d_v1part, NULL, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, (int)c->ntup,
-                            (int)c->nblk, grid_size(c, nitems), c->stream) != 0)
+        ccx_hip_triples(form, axr, axc, bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
+                        c->d_v1part, NULL, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp,
+                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), c->stream) != 0)
         return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[2], c->stream) != 0) return CCX_ERR_DEVICE;
     if (ccx_hip_finalize(c->d_epart, c->d_v1part, c->d_tuples, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,
@@ -322,9 +364,9 @@ int ccx_tuple_w(ccx_ctx *c, int64_t i, int64_t j, int64_t k, double *w)
     DALLOC(d_ep1, c->nblk);
     DALLOC(d_vp, c->nblk * 144);
     if (ccx_hip_memcpy_h2d(d_tup, tup, sizeof(tup), c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
-    if (ccx_hip_triples_f64(c->d_ax, c->d_bx, c->d_voep, c->d_eh, c->d_ep, d_tup, c->d_blocks, d_ep1, d_vp, d_w,
-                            (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, 1, (int)c->nblk,
-                            grid_size(c, c->nblk), c->stream) != 0 ||
+    if (ccx_hip_triples(0, c->d_axr, c->d_axc, c->d_bx, c->d_voep, c->d_eh, c->d_ep, d_tup, c->d_blocks, d_ep1,
+                        d_vp, d_w, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, 1, (int)c->nblk,
+                        grid_size(c, c->nblk), c->stream) != 0 ||
         ccx_hip_memcpy_d2h(w, d_w, sizeof(double) * nu3, c->stream) != 0)
         rc = CCX_ERR_DEVICE;
 fail:
@@ -345,8 +387,15 @@ int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
     if (!c) return CCX_ERR_ARG;
     const int64_t nitems = c->ntup * c->nblk;
     if (items) *items = nitems;
-    /* per item: 36 terms x 16^3 outputs x kp MACs */
-    if (executed_flops) *executed_flops = (double)nitems * 36.0 * 4096.0 * 2.0 * (double)c->kp;
+    /* per item: 6 MFMA terms x 16^3 outputs x kp MACs per accumulated ordering; orderings:
+     * 6 (distinct tuple, A<B<C), 3 (two equal blocks or a degenerate tuple), 1 (A=B=C) */
+    double units = 0.0;
+    for (int64_t t = 0; t < c->ntup; ++t)
+        for (int64_t q = 0; q < c->nblk; ++q) {
+            const int kind = c->h_tuples[4 * t + 3], mult = c->h_blocks[4 * q + 3];
+            units += kind != 1 ? 3.0 : mult == 1 ? 6.0 : mult == 2 ? 3.0 : 1.0;
+        }
+    if (executed_flops) *executed_flops = units * 6.0 * 4096.0 * 2.0 * (double)c->kp;
     return CCX_OK;
 }
 

@@ -14,43 +14,43 @@
 // hole (T2*VM, K=no) contractions are one GEMM over K = nu+no.
 //
 // Work item = (occupied tuple, virtual block orbit).  The virtual cube is cut into
-// 16^3 blocks; a work item owns the blocks (A<=B<=C) and accumulates, for the six
+// 16^3 blocks; a work item owns the blocks (A<=B<=C) and accumulates, for the
 // orderings P of its occupied tuple, w_P(a,b,c) = W_P(a,b,c) over (a,b,c) in A x B x C.
-// Because W_{pi(P)}(pi(abc)) = W_P(abc), these six accumulators hold the values of
-// W_ijk at all six positional permutations of (a,b,c), in the SAME register slot, so
-// the energy and v1 need no transposition of W at all:
-//   sum_s f(s(abc)) = Q(w) / D,  Q = 6 sum w_P^2 + 2 E^2 + 2 O^2 - 8 E O,
+// Because W_{pi.P}(pi.e) = W_P(e) (pi permuting tuple positions and virtual positions
+// alike), the six w_P hold W_ijk at the six positional permutations of (a,b,c), in the
+// SAME register slot, so the energy needs no transposition of W:
+//   sum_s f(s.e) = Q(w) / D,  Q = 6 sum w_P^2 + 2 E^2 + 2 O^2 - 8 E O,
 //   E = w over even orderings, O = w over odd orderings,
-// where f is the summand of t1wt3_ijk's E[T] loop (src/triples-ijk/t1wt3.f90:143-159).
-// The index permutations (trant3, src/common/common_cc.f90:62-333) become the choice
-// of which LDS-staged operand panel a fragment is read from and in which orientation.
+// f being the summand of t1wt3_ijk's E[T] loop (src/triples-ijk/t1wt3.f90:143-159).
+// Degenerate tuples (i,j,j)/(i,i,j) have three distinct orderings; blocks with equal
+// edges (A=B, B=C, A=B=C) get the missing orderings by an in-block transpose, so no
+// ordering is computed twice.  The index permutations (trant3,
+// src/common/common_cc.f90:62-333) become the choice of operand slab and orientation.
 //
-// FP64 MFMA: v_mfma_f64_16x16x4_f64.  Per 16x16 output tile, lane l supplies
-// A[l%16][l/16], B[l/16][l%16] and holds D[l/16 + 4v][l%16], v=0..3 (checked on the device).
+// FP64 MFMA v_mfma_f64_16x16x4_f64: lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// and holds D[l/16 + 4v][l%16], v = 0..3 (checked on the device).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <type_traits>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr int TB = 16;                 // virtual block edge
-constexpr int NTHR = 256;              // 4 waves
+constexpr int NWAVE = 8;               // waves per work group (2 per SIMD)
+constexpr int NTHR = 64 * NWAVE;
+constexpr int BPW = TB / NWAVE;        // batches (16x16 output tiles) per wave = 2
 constexpr int KS = 4;                  // K per MFMA step
-constexpr int PSTR = 17;               // padded alpha stride of an LDS operand panel
-constexpr int PANEL = KS * TB * PSTR;  // 1088 doubles
-constexpr int BXCH = KS * TB;          // 64 doubles
-constexpr int STAGE = 6 * PANEL + 6 * BXCH;
-constexpr int TSTR_B = 17;             // transposition scratch strides
-constexpr int TSTR_A = TB * TSTR_B;
-constexpr int TCUBE = TB * TSTR_A;     // 4352 doubles
+constexpr int CSTR_B = 17;             // padded strides of a 16^3 cube in LDS
+constexpr int CSTR_A = TB * CSTR_B;    // 272
+constexpr int CUBE = TB * CSTR_A;      // 4352 doubles
 constexpr int VSTR = 17;
 constexpr int VBLK = TB * VSTR;        // 272 doubles per staged VOE sub-block
-constexpr int LDS_DOUBLES = (2 * STAGE > 3 * TCUBE) ? ((2 * STAGE > 36 * VBLK) ? 2 * STAGE : 36 * VBLK)
-                                                    : ((3 * TCUBE > 36 * VBLK) ? 3 * TCUBE : 36 * VBLK);
+constexpr int LDS_DOUBLES = (3 * CUBE > 36 * VBLK) ? 3 * CUBE : 36 * VBLK;
 
-// orderings of the occupied tuple positions (0,1,2); parity even for 0,3,4
+// orderings of the occupied tuple positions (0,1,2); even: 0,3,4
 __device__ constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
 __host__ __device__ constexpr int pair_index(int y, int z) { return y * 2 + (z > y ? z - 1 : z); }
@@ -60,8 +60,8 @@ __host__ __device__ constexpr int perm_index(int a, int b, int c)
     return a == 0 ? (b == 1 ? 0 : 1) : a == 1 ? (b == 0 ? 2 : 3) : (b == 0 ? 4 : 5);
 }
 
-// Term tables per phase (0: gamma=b, 1: gamma=c, 2: gamma=a) and accumulator p:
-// panel slot (orientation*3 + tuple position of x) and BX slot (ordered position pair).
+// Term tables per phase (0: gamma=b, 1: gamma=c, 2: gamma=a) and ordering p:
+// operand slot (orientation*3 + tuple position of x) and BX slot (ordered position pair).
 struct Term { int s1, b1, s2, b2; };
 
 __host__ __device__ constexpr Term term(int phase, int p)
@@ -87,9 +87,34 @@ __host__ __device__ constexpr int wsrc(int p, const int *pi)
 
 __host__ __device__ constexpr int wsrc_comp(int p, int s, int t)
 {
-    // position map m -> PERM[s][PERM[t][m]]
     int pi[3] = {PERM[s][PERM[t][0]], PERM[s][PERM[t][1]], PERM[s][PERM[t][2]]};
     return wsrc(p, pi);
+}
+
+// Item modes: which orderings are accumulated.
+enum Mode { M_FULL = 0, M_AAC = 1, M_ACC = 2, M_AAA = 3, M_IJJ = 4, M_IIJ = 5 };
+
+__host__ __device__ constexpr int mode_mask(int mode)
+{
+    return mode == M_FULL ? 0x3f : mode == M_AAC ? 0x0b /*0,1,3*/ : mode == M_ACC ? 0x15 /*0,2,4*/
+         : mode == M_AAA ? 0x01 : mode == M_IJJ ? 0x0d /*0,2,3*/ : 0x13 /*0,1,4*/;
+}
+
+// operand slots a phase needs for the computed orderings
+__host__ __device__ constexpr int slot_mask(int phase, int mode)
+{
+    int m = 0;
+    for (int p = 0; p < 6; ++p)
+        if (mode_mask(mode) & (1 << p)) m |= (1 << term(phase, p).s1) | (1 << term(phase, p).s2);
+    return m;
+}
+
+__host__ __device__ constexpr int bx_mask(int phase, int mode)
+{
+    int m = 0;
+    for (int p = 0; p < 6; ++p)
+        if (mode_mask(mode) & (1 << p)) m |= (1 << term(phase, p).b1) | (1 << term(phase, p).b2);
+    return m;
 }
 
 template <int N, int I = 0, class F>
@@ -101,65 +126,93 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+// The two MFMA forms: operand type T, accumulator tile type, and the row a lane/item
+// of a 16x16 output tile holds (column = lane & 15 for both).
+template <class T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Mfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
 }  // namespace
 
 struct TriplesArgs {
-    const double *ax;      // [no][nb][nb][kp/4][4][16][16]
-    const double *bx;      // [no][no][kp/4][4][nup]
+    const void *axr;       // [no][nb][nb][16 alpha][kp/4][4][16 beta]  (row fragments), type T
+    const void *axc;       // [no][nb][nb][16 beta][kp/4][4][16 alpha]  (column fragments), type T
+    const void *bx;        // [no][no][nb][kp/4][4][16 gamma], type T
     const double *voe;     // [no][no][nup][nup], voe[q][r][x][y] = VOE(x,y,q,r)
     const double *eh;      // [no]
     const double *ep;      // [nup], padded entries finite
-    const int *tuples;     // [ntup][4]: i, j, k, 1 distinct / 2 degenerate
-    const int *blocks;     // [nblk][4]: A, B, C, multiplicity code 1 / 2 / 6
+    const int *tuples;     // [ntup][4]: i, j, k, kind (1 distinct, 2 = (i,j,j), 3 = (i,i,j))
+    const int *blocks;     // [nblk][4]: A, B, C, multiplicity (1, 2 or 6)
     double *e_part;        // [nitems]
     double *v1_part;       // [nitems][3][16][3]
     double *wout;          // W_ijk cube (nu^3, column-major) of tuple 0, or null
+    unsigned int *counter; // work queue head (zeroed before the launch)
     int no, nu, nup, nb, kp, ntup, nblk;
     long long nitems;
 };
 
 // ---------------------------------------------------------------------------------
-// packing kernels (run once per problem)
+// packing kernels (once per problem)
 // ---------------------------------------------------------------------------------
-__global__ void pack_ax_kernel(const double *__restrict__ vvvo, const double *__restrict__ t2,
-                               double *__restrict__ ax, int no, int nu, int nb, int kp)
+__device__ __forceinline__ double ax_value(const double *__restrict__ vvvo, const double *__restrict__ t2,
+                                           int no, int nu, int x, int ag, int bg, long long K)
 {
-    const long long nkb = kp / 4;
-    const long long total = (long long)no * nb * nb * nkb * 1024;
+    if (ag >= nu || bg >= nu) return 0.0;
+    if (K < nu) return vvvo[K + (long long)nu * (bg + (long long)nu * (ag + (long long)nu * x))];
+    if (K < nu + no) return t2[ag + (long long)nu * (bg + (long long)nu * ((K - nu) + (long long)no * x))];
+    return 0.0;
+}
+
+// COLS = 0: row layout [..][alpha][kb][k4][beta]; COLS = 1: column layout [..][beta][kb][k4][alpha]
+template <int COLS, class T>
+__global__ void pack_ax_kernel(const double *__restrict__ vvvo, const double *__restrict__ t2,
+                               T *__restrict__ ax, int no, int nu, int nb, int kp)
+{
+    const long long per_panel = 256LL * kp;
+    const long long total = (long long)no * nb * nb * per_panel;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
-        long long r = idx;
-        const int beta = r & 15; r >>= 4;
-        const int alpha = r & 15; r >>= 4;
-        const int k4 = r & 3; r >>= 2;
-        const long long kb = r % nkb; r /= nkb;
-        const int q = r % nb; r /= nb;
-        const int pblk = r % nb; r /= nb;
-        const int x = (int)r;
-        const int ag = pblk * TB + alpha, bg = q * TB + beta;
-        const long long K = kb * 4 + k4;
-        double v = 0.0;
-        if (ag < nu && bg < nu) {
-            if (K < nu)
-                v = vvvo[K + (long long)nu * (bg + (long long)nu * (ag + (long long)nu * x))];
-            else if (K < nu + no)
-                v = t2[ag + (long long)nu * (bg + (long long)nu * ((K - nu) + (long long)no * x))];
-        }
-        ax[idx] = v;
+        long long r = idx % per_panel;
+        long long pnl = idx / per_panel;
+        const int inner = r & 15;                 // fastest: beta (rows) / alpha (columns)
+        const long long K = (r >> 4) % kp;        // kb*4 + k4
+        const int outer = (int)(r / (16LL * kp)); // alpha (rows) / beta (columns)
+        const int q = pnl % nb; pnl /= nb;
+        const int pblk = pnl % nb; pnl /= nb;
+        const int x = (int)pnl;
+        const int al = COLS ? inner : outer, be = COLS ? outer : inner;
+        ax[idx] = (T)ax_value(vvvo, t2, no, nu, x, pblk * TB + al, q * TB + be, K);
     }
 }
 
+template <class T>
 __global__ void pack_bx_kernel(const double *__restrict__ t2, const double *__restrict__ vm,
-                               double *__restrict__ bx, int no, int nu, int nup, int kp)
+                               T *__restrict__ bx, int no, int nu, int nb, int kp)
 {
-    const long long total = (long long)no * no * kp * nup;
+    const long long total = (long long)no * no * nb * kp * 16;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
         long long r = idx;
-        const int g = r % nup; r /= nup;
+        const int gl = r & 15; r >>= 4;
         const int K = r % kp; r /= kp;
+        const int gb = r % nb; r /= nb;
         const int z = r % no; r /= no;
         const int y = (int)r;
+        const int g = gb * TB + gl;
         double v = 0.0;
         if (g < nu) {
             if (K < nu)
@@ -167,7 +220,7 @@ __global__ void pack_bx_kernel(const double *__restrict__ t2, const double *__re
             else if (K < nu + no)
                 v = -vm[(K - nu) + (long long)no * (g + (long long)nu * (z + (long long)no * y))];
         }
-        bx[idx] = v;
+        bx[idx] = (T)v;
     }
 }
 
@@ -211,103 +264,115 @@ __global__ void unpack_vvvo_kernel(const double *__restrict__ packed, double *__
 // ---------------------------------------------------------------------------------
 namespace {
 
-// Stage one K-step (KS values of K) of a phase's 6 operand panels and 6 BX chunks.
-struct StageRegs {
-    double pa[6][4];
-    double bq[4];
+struct ItemCtx {
+    int tpos[3];        // occupied indices of the tuple positions
+    int blk[3];         // A, B, C
+    int ys[6], zs[6];   // BX slot -> ordered occupied pair
 };
 
-__device__ __forceinline__ void stage_load(const TriplesArgs &g, StageRegs &st, const int *xs,
-                                           int r0o0, int r1o0, int r0o1, int r1o1, int gblk,
-                                           const int *ys, const int *zs, int kb, int tid)
+// Fragment base pointers (wave-uniform) of one phase for this wave's two batches; a
+// fragment of K step kb is base[kb*64 + lane].
+template <class T>
+struct PhasePtrs {
+    const T *p[6][BPW];   // operand slot, batch
+    const T *x[6];        // BX slot
+};
+
+template <int PHASE, class T>
+__device__ __forceinline__ void phase_ptrs(const TriplesArgs &g, const ItemCtx &it, int wave, PhasePtrs<T> &pp)
 {
-    const long long nkb = g.kp / 4;
+    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
+    // orientation 0 / 1 panels (R0 = alpha block, R1 = beta block) and column-read flags
+    int r0[2], r1[2];
+    bool col[2];
+    if (PHASE == 0) { r0[0] = A; r1[0] = C; col[0] = true;  r0[1] = C; r1[1] = A; col[1] = false; }
+    else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
+    else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
+    const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
+    const long long per_panel = 256LL * g.kp;
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int o = s / 3, xp = s % 3;
-        const int r0 = o ? r0o1 : r0o0, r1 = o ? r1o1 : r1o0;
-        const double *src = g.ax + ((((long long)xs[xp] * g.nb + r0) * g.nb + r1) * nkb + kb) * 1024 + tid * 4;
-        const double2 v0 = *reinterpret_cast<const double2 *>(src);
-        const double2 v1 = *reinterpret_cast<const double2 *>(src + 2);
-        st.pa[s][0] = v0.x; st.pa[s][1] = v0.y; st.pa[s][2] = v1.x; st.pa[s][3] = v1.y;
+        const T *base = static_cast<const T *>(col[o] ? g.axc : g.axr) +
+                        (((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o]) * per_panel;
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) pp.p[s][bb] = base + (long long)(wave * BPW + bb) * (16LL * g.kp);
     }
-    if (tid < 96) {
-        const int s = tid >> 4, k = (tid >> 2) & 3, m0 = (tid & 3) * 4;
-        const double *src = g.bx + (((long long)(ys[s] * g.no + zs[s]) * nkb + kb) * 4 + k) * g.nup + gblk * TB + m0;
-        const double2 v0 = *reinterpret_cast<const double2 *>(src);
-        const double2 v1 = *reinterpret_cast<const double2 *>(src + 2);
-        st.bq[0] = v0.x; st.bq[1] = v0.y; st.bq[2] = v1.x; st.bq[3] = v1.y;
-    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        pp.x[q] = static_cast<const T *>(g.bx) + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (16LL * g.kp);
 }
 
-__device__ __forceinline__ void stage_store(double *buf, const StageRegs &st, int tid)
+template <class T>
+struct Frags {
+    T p[6][BPW];
+    T x[6];
+};
+
+template <int PHASE, int MODE, class T>
+__device__ __forceinline__ void load_frags(const PhasePtrs<T> &pp, int kb, int lane, Frags<T> &f)
 {
-    const int e = tid * 4;
-    const int k = e >> 8, al = (e >> 4) & 15, be = e & 15;
+    constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
+    const int off = kb * 64 + lane;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        double *d = buf + s * PANEL + k * (TB * PSTR) + al * PSTR + be;
-        d[0] = st.pa[s][0]; d[1] = st.pa[s][1]; d[2] = st.pa[s][2]; d[3] = st.pa[s][3];
-    }
-    if (tid < 96) {
-        const int s = tid >> 4, k2 = (tid >> 2) & 3, m0 = (tid & 3) * 4;
-        double *d = buf + 6 * PANEL + s * BXCH + k2 * TB + m0;
-        d[0] = st.bq[0]; d[1] = st.bq[1]; d[2] = st.bq[2]; d[3] = st.bq[3];
-    }
+    for (int s = 0; s < 6; ++s)
+        if (SM & (1 << s))
+#pragma unroll
+            for (int bb = 0; bb < BPW; ++bb) f.p[s][bb] = pp.p[s][bb][off];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (XM & (1 << q)) f.x[q] = pp.x[q][off];
 }
 
-// One phase of the K loop.  PHASE 0: gamma=b (batch c), 1: gamma=c (batch b), 2: gamma=a (batch b).
-template <int PHASE>
-__device__ __forceinline__ void run_phase(const TriplesArgs &g, double *lds, d4 (&acc)[6][4],
-                                          const int *xs, const int *ys, const int *zs,
-                                          int A, int B, int C, int tid, int wave, int lane)
+template <int PHASE, int MODE, class T>
+__device__ __forceinline__ void mfma_step(const Frags<T> &f, typename Mfma<T>::acc_t (&acc)[6][BPW])
 {
-    int r0o0, r1o0, r0o1, r1o1, gblk;
-    if (PHASE == 0) { r0o0 = A; r1o0 = C; r0o1 = C; r1o1 = A; gblk = B; }
-    else if (PHASE == 1) { r0o0 = A; r1o0 = B; r0o1 = B; r1o1 = A; gblk = C; }
-    else { r0o0 = B; r1o0 = C; r0o1 = C; r1o1 = B; gblk = A; }
-    // fragment orientation: "row" reads P[k][batch][m], "col" reads P[k][m][batch]
-    constexpr bool row0 = (PHASE == 2);
-    constexpr bool row1 = (PHASE != 2);
-
-    const int nsteps = g.kp / KS;
-    const int kq = lane >> 4, m = lane & 15;
-    StageRegs st;
-    stage_load(g, st, xs, r0o0, r1o0, r0o1, r1o1, gblk, ys, zs, 0, tid);
-    stage_store(lds, st, tid);
-    __syncthreads();
-    for (int s = 0; s < nsteps; ++s) {
-        double *cur = lds + (s & 1) * STAGE;
-        if (s + 1 < nsteps)
-            stage_load(g, st, xs, r0o0, r1o0, r0o1, r1o1, gblk, ys, zs, s + 1, tid);
-        double xf[6];
+    constexpr int MM = mode_mask(MODE);
+    // first term of every computed ordering, then the second, so no MFMA waits on its predecessor
 #pragma unroll
-        for (int q = 0; q < 6; ++q) xf[q] = cur[6 * PANEL + q * BXCH + kq * TB + m];
-#pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            const int bt = wave * 4 + bb;
-            double pf[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const bool row = (q < 3) ? row0 : row1;
-                const double *pn = cur + q * PANEL + kq * (TB * PSTR);
-                pf[q] = row ? pn[bt * PSTR + m] : pn[m * PSTR + bt];
-            }
-            static_for<6>([&](auto P) {
-                constexpr int p = decltype(P)::value;
+    for (int bb = 0; bb < BPW; ++bb)
+        static_for<6>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            if constexpr ((MM >> p) & 1) {
                 constexpr Term t = term(PHASE, p);
-                if constexpr (PHASE == 2) {
-                    acc[p][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[t.b1], pf[t.s1], acc[p][bb], 0, 0, 0);
-                    acc[p][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[t.b2], pf[t.s2], acc[p][bb], 0, 0, 0);
-                } else {
-                    acc[p][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(pf[t.s1], xf[t.b1], acc[p][bb], 0, 0, 0);
-                    acc[p][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(pf[t.s2], xf[t.b2], acc[p][bb], 0, 0, 0);
-                }
-            });
-        }
-        if (s + 1 < nsteps) stage_store(lds + ((s + 1) & 1) * STAGE, st, tid);
-        __syncthreads();
+                if constexpr (PHASE == 2)
+                    acc[p][bb] = Mfma<T>::mac(f.x[t.b1], f.p[t.s1][bb], acc[p][bb]);
+                else
+                    acc[p][bb] = Mfma<T>::mac(f.p[t.s1][bb], f.x[t.b1], acc[p][bb]);
+            }
+        });
+#pragma unroll
+    for (int bb = 0; bb < BPW; ++bb)
+        static_for<6>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            if constexpr ((MM >> p) & 1) {
+                constexpr Term t = term(PHASE, p);
+                if constexpr (PHASE == 2)
+                    acc[p][bb] = Mfma<T>::mac(f.x[t.b2], f.p[t.s2][bb], acc[p][bb]);
+                else
+                    acc[p][bb] = Mfma<T>::mac(f.p[t.s2][bb], f.x[t.b2], acc[p][bb]);
+            }
+        });
+}
+
+// One K sweep.  Register double buffering: fragments of step kb+1 load while step kb runs.
+template <int PHASE, int MODE, class T>
+__device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &it,
+                                          typename Mfma<T>::acc_t (&acc)[6][BPW], int wave, int lane)
+{
+    PhasePtrs<T> pp;
+    phase_ptrs<PHASE, T>(g, it, wave, pp);
+    const int nkb = g.kp / KS;
+    Frags<T> f0, f1;
+    load_frags<PHASE, MODE, T>(pp, 0, lane, f0);
+    int kb = 0;
+    for (; kb + 2 <= nkb; kb += 2) {
+        load_frags<PHASE, MODE, T>(pp, kb + 1, lane, f1);
+        mfma_step<PHASE, MODE, T>(f0, acc);
+        if (kb + 2 < nkb) load_frags<PHASE, MODE, T>(pp, kb + 2, lane, f0);
+        mfma_step<PHASE, MODE, T>(f1, acc);
     }
+    if (kb < nkb) mfma_step<PHASE, MODE, T>(f0, acc);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -324,193 +389,294 @@ __device__ __forceinline__ double group16_sum(double v)
     return v;
 }
 
+// Element (a,b,c) of a 16^3 cube in LDS.
+__device__ __forceinline__ int cidx(int a, int b, int c) { return a * CSTR_A + b * CSTR_B + c; }
+
+// Layout L (after phase 0): wave owns batches b = wave*BPW+bb; lane/v give row a = (lane>>4)+4v,
+// column c = lane&15.  Layout Lb (phase 0): batches c, rows a, columns b.
+template <int MODE, class T>
+__device__ __forceinline__ void transpose_lb_to_l(double *lds, typename Mfma<T>::acc_t (&acc)[6][BPW], int wave,
+                                                  int lane)
+{
+    constexpr int MM = mode_mask(MODE);
+    constexpr int NC = __builtin_popcount(MM);
+    // up to 3 cubes per round
+    static_for<2>([&](auto R) {
+        constexpr int round = decltype(R)::value;
+        if constexpr (round * 3 < NC) {
+            int slot = 0;
+            static_for<6>([&](auto P) {
+                constexpr int p = decltype(P)::value;
+                constexpr int rank = __builtin_popcount(MM & ((1 << p) - 1));
+                if constexpr (((MM >> p) & 1) && rank / 3 == round) {
+                    double *cube = lds + (rank % 3) * CUBE;
+#pragma unroll
+                    for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            cube[cidx(Mfma<T>::row(lane, v), lane & 15, wave * BPW + bb)] = acc[p][bb][v];
+                }
+            });
+            (void)slot;
+            __syncthreads();
+            static_for<6>([&](auto P) {
+                constexpr int p = decltype(P)::value;
+                constexpr int rank = __builtin_popcount(MM & ((1 << p) - 1));
+                if constexpr (((MM >> p) & 1) && rank / 3 == round) {
+                    const double *cube = lds + (rank % 3) * CUBE;
+#pragma unroll
+                    for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            acc[p][bb][v] = cube[cidx(Mfma<T>::row(lane, v), wave * BPW + bb, lane & 15)];
+                }
+            });
+            __syncthreads();
+        }
+    });
+}
+
+// Fill the orderings a mode did not accumulate (layout L).
+template <int MODE, class T>
+__device__ __forceinline__ void complete_orderings(double *lds, typename Mfma<T>::acc_t (&acc)[6][BPW], int wave,
+                                                   int lane)
+{
+    if constexpr (MODE == M_FULL) {
+        return;
+    } else if constexpr (MODE == M_IJJ) {   // (i,j,j): w1 = w0, w4 = w2, w5 = w3
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) { acc[1][bb] = acc[0][bb]; acc[4][bb] = acc[2][bb]; acc[5][bb] = acc[3][bb]; }
+    } else if constexpr (MODE == M_IIJ) {   // (i,i,j): w2 = w0, w3 = w1, w5 = w4
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) { acc[2][bb] = acc[0][bb]; acc[3][bb] = acc[1][bb]; acc[5][bb] = acc[4][bb]; }
+    } else {
+        // Equal blocks: w_P(e) = w_Q(pi.e) read from the cube of Q in LDS.
+        constexpr int MM = mode_mask(MODE);
+        static_for<6>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            constexpr int rank = __builtin_popcount(MM & ((1 << p) - 1));
+            if constexpr ((MM >> p) & 1) {
+                double *cube = lds + rank * CUBE;
+#pragma unroll
+                for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        cube[cidx(Mfma<T>::row(lane, v), wave * BPW + bb, lane & 15)] = acc[p][bb][v];
+            }
+        });
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int e[3] = {Mfma<T>::row(lane, v), wave * BPW + bb, lane & 15};
+                if constexpr (MODE == M_AAC) {         // w2(a,b,c)=w0(b,a,c), w4=w1(b,a,c), w5=w3(b,a,c)
+                    acc[2][bb][v] = lds[0 * CUBE + cidx(e[1], e[0], e[2])];
+                    acc[4][bb][v] = lds[1 * CUBE + cidx(e[1], e[0], e[2])];
+                    acc[5][bb][v] = lds[2 * CUBE + cidx(e[1], e[0], e[2])];
+                } else if constexpr (MODE == M_ACC) {  // w1=w0(a,c,b), w3=w2(a,c,b), w5=w4(a,c,b)
+                    acc[1][bb][v] = lds[0 * CUBE + cidx(e[0], e[2], e[1])];
+                    acc[3][bb][v] = lds[1 * CUBE + cidx(e[0], e[2], e[1])];
+                    acc[5][bb][v] = lds[2 * CUBE + cidx(e[0], e[2], e[1])];
+                } else {                               // A=B=C: w_P(e) = w_0(P^-1 . e)
+                    static_for<5>([&](auto Q) {
+                        constexpr int p = decltype(Q)::value + 1;
+                        constexpr int i0 = PERM[p][0] == 0 ? 0 : PERM[p][1] == 0 ? 1 : 2;
+                        constexpr int i1 = PERM[p][0] == 1 ? 0 : PERM[p][1] == 1 ? 1 : 2;
+                        constexpr int i2 = PERM[p][0] == 2 ? 0 : PERM[p][1] == 2 ? 1 : 2;
+                        acc[p][bb][v] = lds[cidx(e[i0], e[i1], e[i2])];
+                    });
+                }
+            }
+        __syncthreads();
+    }
+}
+
+template <int MODE, class T>
+__device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
+                        int tkind, int mult, int wave, int lane, int tid)
+{
+    typename Mfma<T>::acc_t acc[6][BPW];
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) acc[p][bb] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+
+    run_phase<0, MODE, T>(g, it, acc, wave, lane);
+    transpose_lb_to_l<MODE, T>(lds, acc, wave, lane);
+    run_phase<1, MODE, T>(g, it, acc, wave, lane);
+    run_phase<2, MODE, T>(g, it, acc, wave, lane);
+    complete_orderings<MODE, T>(lds, acc, wave, lane);
+
+    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
+    // ---- stage the 36 VOE sub-blocks: [ordered tuple-position pair][ordered block pair]
+    for (int idx = tid; idx < 36 * 16 * 4; idx += NTHR) {
+        const int quad = idx & 3, row = (idx >> 2) & 15, sb = idx >> 6;
+        const int qr = sb / 6, bpair = sb % 6;
+        const int y = qr >> 1, z = (qr & 1) ? (y == 2 ? 1 : 2) : (y == 0 ? 1 : 0);
+        const int ry = bpair >> 1, rz = (bpair & 1) ? (ry == 2 ? 1 : 2) : (ry == 0 ? 1 : 0);
+        const double *src = g.voe + (((long long)it.tpos[y] * g.no + it.tpos[z]) * g.nup + it.blk[ry] * TB + row) * g.nup
+                            + it.blk[rz] * TB + quad * 4;
+        double *d = lds + sb * VBLK + row * VSTR + quad * 4;
+        d[0] = src[0]; d[1] = src[1]; d[2] = src[2]; d[3] = src[3];
+    }
+    __syncthreads();
+
+    // ---- epilogue: energy, v1 partials, optional W store
+    const double dijk = g.eh[it.tpos[0]] + g.eh[it.tpos[1]] + g.eh[it.tpos[2]];
+    const double wgt = (tkind == 1 ? 1.0 : 0.5) / (double)mult;
+    double esum = 0.0;
+    double va[4][3], vb[BPW][3], vc[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) va[i][p] = 0.0;
+#pragma unroll
+    for (int i = 0; i < BPW; ++i)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) vb[i][p] = 0.0;
+    vc[0] = vc[1] = vc[2] = 0.0;
+    const int cl = lane & 15;
+    const int cg = C * TB + cl;
+#pragma unroll
+    for (int bb = 0; bb < BPW; ++bb) {
+        const int bl = wave * BPW + bb;
+        const int bg = B * TB + bl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int al = Mfma<T>::row(lane, v);
+            const int ag = A * TB + al;
+            if (ag >= g.nu || bg >= g.nu || cg >= g.nu) continue;
+            double w[6];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) w[p] = (double)acc[p][bb][v];
+            const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
+            const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
+            const double sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3] + w[4] * w[4] + w[5] * w[5];
+            esum += (6.0 * sq + 2.0 * ev * ev + 2.0 * od * od - 8.0 * ev * od) * inv;
+            const int el[3] = {al, bl, cl};
+            static_for<6>([&](auto S) {
+                constexpr int s = decltype(S)::value;
+                constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
+                constexpr int bpair = pair_index(r1, r2);
+                static_for<6>([&](auto PP) {
+                    constexpr int p = decltype(PP)::value;
+                    constexpr int q0 = wsrc_comp(p, s, 0);   // sigma
+                    constexpr int q1 = wsrc_comp(p, s, 2);   // sigma o (1,0,2)
+                    constexpr int q2 = wsrc_comp(p, s, 1);   // sigma o (0,2,1)
+                    constexpr int q3 = wsrc_comp(p, s, 3);   // sigma o (1,2,0)
+                    constexpr int qr = pair_index(PERM[p][1], PERM[p][2]);
+                    const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
+                    const double vv = lds[(qr * 6 + bpair) * VBLK + el[r1] * VSTR + el[r2]];
+                    const double c = z * vv * inv;
+                    if constexpr (r0 == 0) va[v][PERM[p][0]] += c;
+                    else if constexpr (r0 == 1) vb[bb][PERM[p][0]] += c;
+                    else vc[PERM[p][0]] += c;
+                });
+            });
+            if (g.wout != nullptr) {
+                const int xg[3] = {ag, bg, cg};
+                static_for<6>([&](auto S) {
+                    constexpr int s = decltype(S)::value;
+                    constexpr int q0 = wsrc_comp(0, s, 0);
+                    g.wout[xg[PERM[s][0]] + (long long)g.nu * (xg[PERM[s][1]] + (long long)g.nu * xg[PERM[s][2]])] = w[q0];
+                });
+            }
+        }
+    }
+
+    // ---- deterministic reductions
+    double *rw = red[wave];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)          // role a: rows (lane>>4)+4v
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double s = group16_sum(va[v][p]);
+            if ((lane & 15) == 0) rw[Mfma<T>::row(lane, v) * 3 + p] = s;
+        }
+#pragma unroll
+    for (int bb = 0; bb < BPW; ++bb)     // role b: rows wave*BPW+bb
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double s = wave_sum(vb[bb][p]);
+            if (lane == 0) rw[48 + bb * 3 + p] = s;
+        }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {        // role c: rows lane&15
+        double s = vc[p];
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (lane < 16) rw[54 + lane * 3 + p] = s;
+    }
+    {
+        const double s = wave_sum(esum);
+        if (lane == 0) rw[102] = s;
+    }
+    __syncthreads();
+    double *vp = g.v1_part + item * 144;
+    if (tid < 48) {
+        double s = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][tid];
+        vp[tid] = s * wgt;
+    } else if (tid < 96) {
+        const int r = tid - 48, row = r / 3, p = r % 3;
+        vp[48 + r] = red[row / BPW][48 + (row % BPW) * 3 + p] * wgt;
+    } else if (tid < 144) {
+        const int r = tid - 96;
+        double s = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][54 + r];
+        vp[96 + r] = s * wgt;
+    } else if (tid == 144) {
+        double s = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][102];
+        g.e_part[item] = s * wgt;
+    }
+    __syncthreads();
+}
+
 }  // namespace
 
-__global__ void __launch_bounds__(NTHR, 1) triples_f64_kernel(TriplesArgs g)
+template <class T>
+__global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
 {
     extern __shared__ double lds[];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    __shared__ double red[4][16 * 3 * 4 + 16 * 3 + 16 * 3 + 1];
+    __shared__ double red[NWAVE][112];
+    __shared__ long long next_item;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-    for (long long item = blockIdx.x; item < g.nitems; item += gridDim.x) {
-        const int t = (int)(item / g.nblk), q = (int)(item % g.nblk);
-        const int4 tp = reinterpret_cast<const int4 *>(g.tuples)[t];
-        const int4 bp = reinterpret_cast<const int4 *>(g.blocks)[q];
-        const int tpos[3] = {tp.x, tp.y, tp.z};
-        const int A = bp.x, B = bp.y, C = bp.z;
-        // BX slots: ordered pairs of tuple positions
-        int ys[6], zs[6];
+    for (;;) {
+        if (tid == 0) next_item = (long long)atomicAdd(g.counter, 1u);
+        __syncthreads();
+        const long long item = next_item;
+        __syncthreads();
+        if (item >= g.nitems) break;
+        const int t = __builtin_amdgcn_readfirstlane((int)(item / g.nblk));
+        const int q = __builtin_amdgcn_readfirstlane((int)(item % g.nblk));
+        int4 tp = reinterpret_cast<const int4 *>(g.tuples)[t];
+        int4 bp = reinterpret_cast<const int4 *>(g.blocks)[q];
+        tp.x = __builtin_amdgcn_readfirstlane(tp.x); tp.y = __builtin_amdgcn_readfirstlane(tp.y);
+        tp.z = __builtin_amdgcn_readfirstlane(tp.z); tp.w = __builtin_amdgcn_readfirstlane(tp.w);
+        bp.x = __builtin_amdgcn_readfirstlane(bp.x); bp.y = __builtin_amdgcn_readfirstlane(bp.y);
+        bp.z = __builtin_amdgcn_readfirstlane(bp.z); bp.w = __builtin_amdgcn_readfirstlane(bp.w);
+        ItemCtx it;
+        it.tpos[0] = tp.x; it.tpos[1] = tp.y; it.tpos[2] = tp.z;
+        it.blk[0] = bp.x; it.blk[1] = bp.y; it.blk[2] = bp.z;
 #pragma unroll
         for (int y = 0; y < 3; ++y)
 #pragma unroll
             for (int z = 0; z < 3; ++z)
-                if (y != z) { ys[pair_index(y, z)] = tpos[y]; zs[pair_index(y, z)] = tpos[z]; }
-
-        d4 acc[6][4];
-#pragma unroll
-        for (int p = 0; p < 6; ++p)
-#pragma unroll
-            for (int bb = 0; bb < 4; ++bb) acc[p][bb] = d4{0.0, 0.0, 0.0, 0.0};
-
-        // phase gamma=b in layout (batch c; rows a, cols b)
-        run_phase<0>(g, lds, acc, tpos, ys, zs, A, B, C, tid, wave, lane);
-
-        // transpose to layout (batch b; rows a, cols c) through LDS, 3 accumulators at a time
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-#pragma unroll
-            for (int pp = 0; pp < 3; ++pp)
-#pragma unroll
-                for (int bb = 0; bb < 4; ++bb)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int a = (lane >> 4) + 4 * v, b = lane & 15, c = wave * 4 + bb;
-                        lds[pp * TCUBE + a * TSTR_A + b * TSTR_B + c] = acc[half * 3 + pp][bb][v];
-                    }
-            __syncthreads();
-#pragma unroll
-            for (int pp = 0; pp < 3; ++pp)
-#pragma unroll
-                for (int bb = 0; bb < 4; ++bb)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int a = (lane >> 4) + 4 * v, b = wave * 4 + bb, c = lane & 15;
-                        acc[half * 3 + pp][bb][v] = lds[pp * TCUBE + a * TSTR_A + b * TSTR_B + c];
-                    }
-            __syncthreads();
-        }
-
-        run_phase<1>(g, lds, acc, tpos, ys, zs, A, B, C, tid, wave, lane);
-        run_phase<2>(g, lds, acc, tpos, ys, zs, A, B, C, tid, wave, lane);
-
-        // ---- stage the 36 VOE sub-blocks: [ordered tuple-position pair][ordered block pair]
-        const int blk3[3] = {A, B, C};
-        for (int idx = tid; idx < 36 * 16 * 4; idx += NTHR) {
-            const int quad = idx & 3, row = (idx >> 2) & 15, sb = idx >> 6;
-            const int qr = sb / 6, bpair = sb % 6;
-            int y = 0, z = 0, ry = 0, rz = 0;
-#pragma unroll
-            for (int yy = 0; yy < 3; ++yy)
-#pragma unroll
-                for (int zz = 0; zz < 3; ++zz)
-                    if (yy != zz) {
-                        if (pair_index(yy, zz) == qr) { y = yy; z = zz; }
-                        if (pair_index(yy, zz) == bpair) { ry = yy; rz = zz; }
-                    }
-            const double *src = g.voe + (((long long)tpos[y] * g.no + tpos[z]) * g.nup + blk3[ry] * TB + row) * g.nup
-                                + blk3[rz] * TB + quad * 4;
-            double *d = lds + sb * VBLK + row * VSTR + quad * 4;
-            d[0] = src[0]; d[1] = src[1]; d[2] = src[2]; d[3] = src[3];
-        }
-        __syncthreads();
-
-        // ---- epilogue: energy, v1 partials, optional W store
-        const double dijk = g.eh[tp.x] + g.eh[tp.y] + g.eh[tp.z];
-        const double wgt = (tp.w == 1 ? 1.0 : 0.5) / (double)bp.w;
-        double esum = 0.0;
-        double va[4][3], vb[4][3], vc[3];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int p = 0; p < 3; ++p) { va[i][p] = 0.0; vb[i][p] = 0.0; }
-        vc[0] = vc[1] = vc[2] = 0.0;
-        const int cl = lane & 15;
-        const int cg = C * TB + cl;
-#pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            const int bl = wave * 4 + bb;
-            const int bg = B * TB + bl;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int al = (lane >> 4) + 4 * v;
-                const int ag = A * TB + al;
-                if (ag >= g.nu || bg >= g.nu || cg >= g.nu) continue;
-                double w[6];
-#pragma unroll
-                for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
-                const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
-                const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
-                const double sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3] + w[4] * w[4] + w[5] * w[5];
-                esum += (6.0 * sq + 2.0 * ev * ev + 2.0 * od * od - 8.0 * ev * od) * inv;
-                const int el[3] = {al, bl, cl};
-                const double winv = inv;
-                static_for<6>([&](auto S) {
-                    constexpr int s = decltype(S)::value;
-                    constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
-                    constexpr int bpair = pair_index(r1, r2);
-                    static_for<6>([&](auto PP) {
-                        constexpr int p = decltype(PP)::value;
-                        constexpr int q0 = wsrc_comp(p, s, 0);   // sigma
-                        constexpr int q1 = wsrc_comp(p, s, 2);   // sigma o (1,0,2)
-                        constexpr int q2 = wsrc_comp(p, s, 1);   // sigma o (0,2,1)
-                        constexpr int q3 = wsrc_comp(p, s, 3);   // sigma o (1,2,0)
-                        constexpr int qr = pair_index(PERM[p][1], PERM[p][2]);
-                        const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
-                        const double vv = lds[(qr * 6 + bpair) * VBLK + el[r1] * VSTR + el[r2]];
-                        const double c = z * vv * winv;
-                        if constexpr (r0 == 0) va[v][PERM[p][0]] += c;
-                        else if constexpr (r0 == 1) vb[bb][PERM[p][0]] += c;
-                        else vc[PERM[p][0]] += c;
-                    });
-                });
-                if (g.wout != nullptr && t == 0) {
-                    const int xg[3] = {ag, bg, cg};
-                    static_for<6>([&](auto S) {
-                        constexpr int s = decltype(S)::value;
-                        constexpr int q0 = wsrc_comp(0, s, 0);
-                        g.wout[xg[PERM[s][0]] + (long long)g.nu * (xg[PERM[s][1]] + (long long)g.nu * xg[PERM[s][2]])] = w[q0];
-                    });
-                }
-            }
-        }
-        __syncthreads();   // VOE staging no longer needed; reuse red[]
-
-        // ---- deterministic reductions
-        double *rw = red[wave];
-        // role a: rows (lane>>4)+4v, reduce over the 16 lanes of a group, then waves
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                const double s = group16_sum(va[v][p]);
-                if ((lane & 15) == 0) rw[((lane >> 4) + 4 * v) * 3 + p] = s;
-            }
-        // role b: rows wave*4+bb, whole-wave sums
-#pragma unroll
-        for (int bb = 0; bb < 4; ++bb)
-#pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                const double s = wave_sum(vb[bb][p]);
-                if (lane == 0) rw[48 + bb * 3 + p] = s;
-            }
-        // role c: rows lane&15, sum over the 4 lane groups, then waves
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            double s = vc[p];
-            s += __shfl_xor(s, 16, 64);
-            s += __shfl_xor(s, 32, 64);
-            if (lane < 16) rw[60 + lane * 3 + p] = s;
-        }
-        {
-            const double s = wave_sum(esum);
-            if (lane == 0) rw[108] = s;
-        }
-        __syncthreads();
-        double *vp = g.v1_part + item * 144;
-        if (tid < 48) {            // role a
-            const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-            vp[tid] = s * wgt;
-        } else if (tid < 96) {     // role b: row = tid-48 -> wave (row/4), bb (row%4)
-            const int r = tid - 48, row = r / 3, p = r % 3;
-            vp[48 + r] = red[row >> 2][48 + (row & 3) * 3 + p] * wgt;
-        } else if (tid < 144) {    // role c
-            const int r = tid - 96;
-            const double s = red[0][60 + r] + red[1][60 + r] + red[2][60 + r] + red[3][60 + r];
-            vp[96 + r] = s * wgt;
-        } else if (tid == 144) {
-            g.e_part[item] = (red[0][108] + red[1][108] + red[2][108] + red[3][108]) * wgt;
-        }
-        __syncthreads();
+                if (y != z) { it.ys[pair_index(y, z)] = it.tpos[y]; it.zs[pair_index(y, z)] = it.tpos[z]; }
+        const int tkind = tp.w, mult = bp.w;
+        if (tkind == 2) do_item<M_IJJ, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (tkind == 3) do_item<M_IIJ, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (mult == 1) do_item<M_FULL, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (mult == 6) do_item<M_AAA, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (bp.x == bp.y) do_item<M_AAC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else do_item<M_ACC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
     }
 }
 
@@ -597,6 +763,14 @@ __global__ void __launch_bounds__(1024) finalize_energy_kernel(const double *__r
     }
 }
 
+__global__ void zero_counter_kernel(unsigned int *c) { *c = 0u; }
+
+__global__ void to_f32_kernel(const double *__restrict__ src, float *__restrict__ dst, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
 // ---------------------------------------------------------------------------------
 // thin C ABI used by the C host layer
 // ---------------------------------------------------------------------------------
@@ -652,38 +826,53 @@ static int grid_for(long long total)
     return (int)g;
 }
 
-int ccx_hip_pack(const double *vvvo, const double *t2, const double *vm, const double *voe, double *ax,
-                 double *bx, double *voep, int no, int nu, int nup, int kp, void *stream)
+// form 0: FP64 slabs (double); form 1: FP32 slabs (float).  voep is written only when non-null.
+int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *vm, const double *voe, void *axr,
+                 void *axc, void *bx, double *voep, int no, int nu, int nup, int kp, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     const int nb = nup / TB;
-    pack_ax_kernel<<<grid_for((long long)no * nb * nb * kp * 256), 256, 0, s>>>(vvvo, t2, ax, no, nu, nb, kp);
-    pack_bx_kernel<<<grid_for((long long)no * no * kp * nup), 256, 0, s>>>(t2, vm, bx, no, nu, nup, kp);
-    pack_voe_kernel<<<grid_for((long long)no * no * nup * nup), 256, 0, s>>>(voe, voep, no, nu, nup);
+    const long long nax = (long long)no * nb * nb * kp * 256;
+    const int gbx = grid_for((long long)no * no * nb * kp * 16);
+    if (form == 0) {
+        pack_ax_kernel<0, double><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (double *)axr, no, nu, nb, kp);
+        pack_ax_kernel<1, double><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (double *)axc, no, nu, nb, kp);
+        pack_bx_kernel<double><<<gbx, 256, 0, s>>>(t2, vm, (double *)bx, no, nu, nb, kp);
+    } else {
+        pack_ax_kernel<0, float><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (float *)axr, no, nu, nb, kp);
+        pack_ax_kernel<1, float><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (float *)axc, no, nu, nb, kp);
+        pack_bx_kernel<float><<<gbx, 256, 0, s>>>(t2, vm, (float *)bx, no, nu, nb, kp);
+    }
+    if (voep) pack_voe_kernel<<<grid_for((long long)no * no * nup * nup), 256, 0, s>>>(voe, voep, no, nu, nup);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int ccx_hip_triples_f64(const double *ax, const double *bx, const double *voep, const double *eh,
-                        const double *ep, const int *tuples, const int *blocks, double *e_part, double *v1_part,
-                        double *wout, int no, int nu, int nup, int kp, int ntup, int nblk, int grid,
-                        void *stream)
+int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
+                    const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
+                    double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
+                    int ntup, int nblk, int grid, void *stream)
 {
-    static bool attr_set = false;
+    static bool attr_set[2] = {false, false};
     const size_t lds_bytes = sizeof(double) * LDS_DOUBLES;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)triples_f64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_bytes) != hipSuccess)
+    if (form != 0 && form != 1) return 3;
+    if (!attr_set[form]) {
+        const void *fn = form == 0 ? (const void *)triples_kernel<double> : (const void *)triples_kernel<float>;
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return 2;
-        attr_set = true;
+        attr_set[form] = true;
     }
     TriplesArgs g;
-    g.ax = ax; g.bx = bx; g.voe = voep; g.eh = eh; g.ep = ep; g.tuples = tuples; g.blocks = blocks;
-    g.e_part = e_part; g.v1_part = v1_part; g.wout = wout;
+    g.axr = axr; g.axc = axc; g.bx = bx; g.voe = voep; g.eh = eh; g.ep = ep; g.tuples = tuples;
+    g.blocks = blocks; g.e_part = e_part; g.v1_part = v1_part; g.wout = wout; g.counter = counter;
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
     if (g.nitems == 0) return 0;
-    if (grid <= 0 || grid > g.nitems) grid = (int)(g.nitems < 256 ? g.nitems : 256);
-    triples_f64_kernel<<<grid, NTHR, lds_bytes, (hipStream_t)stream>>>(g);
+    if (grid <= 0) grid = 256;
+    if (grid > g.nitems) grid = (int)g.nitems;
+    hipStream_t s = (hipStream_t)stream;
+    zero_counter_kernel<<<1, 1, 0, s>>>(counter);
+    if (form == 0) triples_kernel<double><<<grid, NTHR, lds_bytes, s>>>(g);
+    else triples_kernel<float><<<grid, NTHR, lds_bytes, s>>>(g);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -704,6 +893,12 @@ int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tup
 }
 
 int ccx_hip_lds_bytes(void) { return (int)(sizeof(double) * LDS_DOUBLES); }
+
+int ccx_hip_to_f32(const double *src, float *dst, long long n, void *stream)
+{
+    to_f32_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(src, dst, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 int ccx_hip_unpack_vvvo(const double *packed, double *full, int no, int nu, void *stream)
 {

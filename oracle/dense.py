@@ -129,3 +129,43 @@ def textbook(no: int, nu: int, eh, ep, t1, eri) -> dict:
     e4 = (left * (w - _vp(w, "cba")) / (3.0 * d)).sum()
     est = (left * (y - _vp(y, "cba")) / (3.0 * d)).sum()
     return {"etd": float(e4), "ets": float(est), "e": float(e4 + est)}
+
+
+def grad_dense(inp: dict) -> dict:
+    """dE(T)/d(T1, T2, VM, VOE, VVVO) by torch autograd (float64, CPU) through the dense
+    closed form above; T2 and VOE are independent inputs.  Small shapes only."""
+    import torch
+
+    f64 = torch.float64
+    leaves = {k: torch.tensor(np.ascontiguousarray(inp[k]), dtype=f64, requires_grad=True)
+              for k in ("t1", "t2", "vm", "voe", "vvvo")}
+    eh = torch.tensor(inp["eh"], dtype=f64)
+    ep = torch.tensor(inp["ep"], dtype=f64)
+    nu = inp["nu"]
+    part = torch.einsum("dbai,cdjk->ijkabc", leaves["vvvo"], leaves["t2"])
+    hole = torch.einsum("abli,lckj->ijkabc", leaves["t2"], leaves["vm"])
+    p = part - hole
+    w = torch.zeros_like(p)
+    for s in _PERMS:
+        w = w + p.permute(*s, *[3 + x for x in s])
+    o = eh[:, None, None] + eh[None, :, None] + eh[None, None, :]
+    v = ep[:, None, None] + ep[None, :, None] + ep[None, None, :]
+    d = o[:, :, :, None, None, None] - v[None, None, None]
+
+    def vp(s):
+        return torch.einsum("ijk" + s + "->ijkabc", w)
+
+    f = 8 * w - 4 * (vp("acb") + vp("cba") + vp("bac")) + 2 * (vp("bca") + vp("cab"))
+    term = w * f / d
+    mask = torch.ones_like(term)
+    a = torch.arange(nu)
+    mask[:, :, :, a, a, a] = 0
+    etd = (term * mask).sum() / 6.0
+    z = (2 * w - 2 * vp("bac") - vp("acb") + vp("bca")) / d
+    v1 = torch.einsum("pqrabc,bcqr->ap", z, leaves["voe"])
+    ets = 2 * torch.einsum("pa,ap->", leaves["t1"], v1)
+    e = etd + ets
+    e.backward()
+    out = {k: np.asfortranarray(leaves[k].grad.numpy()) for k in leaves}
+    out["e"] = float(e)
+    return out

@@ -42,6 +42,7 @@ def wsrc_comp(p: int, s: int, t: int) -> int:
 
 
 def tuples(no: int) -> list:
+    """kind 1 = distinct, 2 = (i,j,j), 3 = (i,i,j)"""
     out = []
     for i in range(no):
         for j in range(i):
@@ -50,8 +51,44 @@ def tuples(no: int) -> list:
     for i in range(no):
         for j in range(i):
             out.append((i, j, j, 2))
-            out.append((i, i, j, 2))
+            out.append((i, i, j, 3))
     return out
+
+
+# orderings each item mode accumulates (the kernel's mode_mask)
+MODE_SET = {"full": (0, 1, 2, 3, 4, 5), "aac": (0, 1, 3), "acc": (0, 2, 4), "aaa": (0,),
+            "ijj": (0, 2, 3), "iij": (0, 1, 4)}
+
+
+def item_mode(kind: int, mult: int, A: int, B: int) -> str:
+    if kind == 2:
+        return "ijj"
+    if kind == 3:
+        return "iij"
+    if mult == 1:
+        return "full"
+    if mult == 6:
+        return "aaa"
+    return "aac" if A == B else "acc"
+
+
+def complete(acc: np.ndarray, mode: str) -> np.ndarray:
+    """Fill the orderings a mode did not accumulate (the kernel's complete_orderings)."""
+    w = acc.copy()
+    if mode == "ijj":
+        w[1], w[4], w[5] = w[0], w[2], w[3]
+    elif mode == "iij":
+        w[2], w[3], w[5] = w[0], w[1], w[4]
+    elif mode == "aac":                 # w2(a,b,c)=w0(b,a,c), w4=w1(b,a,c), w5=w3(b,a,c)
+        w[2], w[4], w[5] = (np.swapaxes(acc[q], 0, 1) for q in (0, 1, 3))
+    elif mode == "acc":                 # w1=w0(a,c,b), w3=w2(a,c,b), w5=w4(a,c,b)
+        w[1], w[3], w[5] = (np.swapaxes(acc[q], 1, 2) for q in (0, 2, 4))
+    elif mode == "aaa":                 # w_P(e) = w_0(P^-1 . e)
+        for p in range(1, 6):
+            inv = [PERM[p].index(m) for m in range(3)]
+            # out[e0,e1,e2] = acc0[e_inv0, e_inv1, e_inv2]
+            w[p] = np.einsum("xyz->" + "".join("xyz"[inv.index(n)] for n in range(3)), acc[0])
+    return w
 
 
 def blocks(nb: int) -> list:
@@ -87,7 +124,8 @@ def model(inp: dict, tb: int = 4) -> dict:
         for (A, B, C, mult) in blocks(nb):
             sa, sb, sc = (slice(X * tb, X * tb + tb) for X in (A, B, C))
             acc = np.zeros((6, tb, tb, tb))
-            for p in range(6):
+            mode = item_mode(kind, mult, A, B)
+            for p in MODE_SET[mode]:
                 for phase in range(3):
                     s1, b1, s2, b2 = term(phase, p)
                     for (s, bq) in ((s1, b1), (s2, b2)):
@@ -105,6 +143,7 @@ def model(inp: dict, tb: int = 4) -> dict:
                         else:               # gamma=a
                             pan = ax[x][sb, sc] if o == 0 else np.swapaxes(ax[x][sc, sb], 0, 1)
                             acc[p] += np.einsum("bck,ak->abc", pan, bxm[sa])
+            acc = complete(acc, mode)
             wgt = (1.0 if kind == 1 else 0.5) / mult
             dijk = eh[ti] + eh[tj] + eh[tk]
             for al in range(tb):

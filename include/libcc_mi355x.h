@@ -82,6 +82,17 @@ int ccx_last_timing(const ccx_ctx *ctx, double *kernel_ms, double *total_ms);
 /* Work items and work-item FLOPs of the persistent launch (for roofline accounting). */
 int ccx_work_stats(const ccx_ctx *ctx, int64_t *items, double *executed_flops);
 
+/* Adjoint of E(T) = etd + ets with respect to every input array (FP64), by re-running the
+ * tuple loop: dE/dT1 (no,nu), dE/dT2 (nu,nu,no,no), dE/dVM (no,nu,no,no), dE/dVOE
+ * (nu,nu,no,no) and dE/dVVVO (nu,nu,nu,no, unpacked; each entry an independent variable)
+ * in the reference layouts; T2 and VOE are treated as independent inputs.  Any gradient
+ * pointer may be NULL; on_device selects device or host pointers.  Also returns the
+ * energies.  The reference has no adjoint; the energy it differentiates is
+ * src/triples-ijk/triples.F90:29-313 (W from ijk_tuple.f90:56-84, E[T] and v1 from
+ * t1wt3.f90:53-102, ets = 2 T1.v1 from triples.F90:310-313). */
+int ccx_gradient(ccx_ctx *ctx, double *ets, double *etd, double *dt1, double *dt2, double *dvm, double *dvoe,
+                 double *dvvvo, int on_device);
+
 /* Set the stream (hipStream_t cast to void*) later calls use; NULL = default stream. */
 int ccx_set_stream(ccx_ctx *ctx, void *stream);
 

@@ -58,6 +58,8 @@ def lib() -> ctypes.CDLL:
     L.ccx_last_timing.restype = ctypes.c_int
     L.ccx_work_stats.argtypes = [ctxp, ctypes.POINTER(i64), dp]
     L.ccx_work_stats.restype = ctypes.c_int
+    L.ccx_gradient.argtypes = [ctxp, dp, dp] + [vp] * 5 + [ctypes.c_int]
+    L.ccx_gradient.restype = ctypes.c_int
     L.ccx_set_stream.argtypes = [ctxp, vp]
     L.ccx_set_stream.restype = ctypes.c_int
     pi64 = ctypes.POINTER(i64)

@@ -1,0 +1,865 @@
+// Adjoint of the closed-shell (T) energy on gfx950: dE/d(T1, T2, VM, VOE, VVVO).
+//
+// E = E[T] + E_ST with (per tuple t of weight wt, canonical cube W_t)
+//   E[T] = sum_t wt sum_abc X_t,   X = W (8W - 4(W_acb+W_cba+W_bac) + 2(W_bca+W_cab)) / D
+//   E_ST = 2 sum T1 . v1,          v1(a,p) = sum Z_pqr(a,b,c) VOE(b,c,q,r)
+// (src/triples-ijk/t1wt3.f90:53-102, src/triples-ijk/triples.F90:310-313).  The tuple
+// loop is re-run: W is recomputed per work item exactly as in the forward kernel and
+// never stored across tuples.  Per batch of tuples:
+//
+//  1. adj_item_kernel  (tuple, block orbit) items: W in registers, then in the epilogue
+//     Wbar_t(x) = dE/dW_t(x) at the six positional permutations of every point,
+//       Wbar = wt [ (12 W_s + 4 E' - 8 O') / D  +  2/D sum_{p,mu} c_mu T1.VOE ],
+//     written to the batch's blocked Wbar cube; the T1.Z contraction for dVOE reduced
+//     over the item's a-block into per-(tuple, ordering, block triple) slots; energy and
+//     v1 partials as the forward kernel.
+//  2. gax_kernel       gAX[x](ab,K) += sum_g Wbar_sigma(a,b,g) BX[y,z](g,K)
+//  3. gbx_kernel       gBX[y,z](g,K) += sum_ab Wbar_sigma(a,b,g) AX[x](a,b,K)
+//     (sigma over the six simultaneous permutations that build W from AX and BX).
+//  4. dvoe_reduce_kernel  slots -> dVOE.
+// After the last batch unpack_grad_kernel maps gAX/gBX back to VVVO, T2 and VM.
+// Every accumulation is in a fixed order: the result is bitwise reproducible.
+#include "ccx_item.h"
+
+namespace {
+
+constexpr int WB_STR_A = 273;                      // padded strides of a 16^3 Wbar block in LDS
+constexpr int WB_STR_B = 17;
+constexpr int WB_BLK = TB * WB_STR_A;              // 4368 doubles
+constexpr int ADJ_V = 0;                           // LDS regions of the item epilogue (doubles)
+constexpr int ADJ_D = 36 * VBLK;                   // 36 staged VOE sub-blocks
+constexpr int ADJ_T = ADJ_D + 36 * 256;            // 36 dVOE partial sub-blocks
+constexpr int ADJ_END = ADJ_T + 144;               // T1 rows of the tuple over the item's blocks
+constexpr int ADJ_LDS = ADJ_END > 3 * CUBE ? ADJ_END : 3 * CUBE;
+
+// Z terms of the ets adjoint: Z(x) = sum_m c_m W(mu_m x) / D with
+// mu = id, (1,0,2), (0,2,1), (1,2,0) and c = 2, -2, -1, 1.
+__device__ constexpr int ZMU[4][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 2, 0}};
+__device__ constexpr double ZC[4] = {2.0, -2.0, -1.0, 1.0};
+
+__host__ __device__ constexpr int inv_perm_at(const int *p, int n) { return p[0] == n ? 0 : p[1] == n ? 1 : 2; }
+
+// Placement index of the T1.VOE product that Z term m of ordering p contributes to
+// Wbar at placement s: n = PERM[s] o lam^-1 with lam(n') = mu_m(p^-1(n')).
+__host__ __device__ constexpr int sn_index(int s, int p, int m)
+{
+    int lam[3] = {0, 0, 0};
+    for (int n = 0; n < 3; ++n) lam[n] = ZMU[m][inv_perm_at(PERM[p], n)];
+    int li[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) li[k] = inv_perm_at(lam, k);
+    return perm_index(PERM[s][li[0]], PERM[s][li[1]], PERM[s][li[2]]);
+}
+
+// index of PERM[s]^-1
+__host__ __device__ constexpr int perm_inverse(int s)
+{
+    return perm_index(inv_perm_at(PERM[s], 0), inv_perm_at(PERM[s], 1), inv_perm_at(PERM[s], 2));
+}
+
+// sigma-terms: W_t(e) += sum_K AX[t[xp]](e_u, e_v, K) BX[t[yp], t[zp]](e_wf, K)
+struct Sigma { int xp, u, v, yp, zp, wf; };
+__device__ constexpr Sigma SIG[6] = {{0, 0, 1, 1, 2, 2}, {1, 1, 0, 0, 2, 2}, {2, 2, 1, 1, 0, 0},
+                                     {0, 0, 2, 2, 1, 1}, {1, 1, 2, 2, 0, 0}, {2, 2, 0, 0, 1, 1}};
+
+}  // namespace
+
+struct AdjArgs {
+    TriplesArgs f;          // forward operands; e_part / v1_part indexed by global item
+    const double *t1;       // T1(no, nu), column-major
+    double *wbar;           // [tcount][nb][nb][nb][16][16][16]
+    double *slots;          // [tcount][6][nb][nb][nb][16][16]  dVOE partials
+    int t0, tcount;         // tuples of this batch
+};
+
+namespace {
+
+// Epilogue of one adjoint work item; acc holds w_p in layout L.
+template <int MODE>
+__device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112], const ItemCtx &it,
+                             long long item, int tl, int tkind, int mult, d4 (&acc)[6][BPW], int wave, int lane,
+                             int tid)
+{
+    const TriplesArgs &g = ga.f;
+    const int nb = g.nb;
+    double *V = lds + ADJ_V, *D = lds + ADJ_D, *T = lds + ADJ_T;
+    // ---- stage: 36 VOE sub-blocks (as the forward epilogue), T1 rows, zero dVOE partials
+    for (int idx = tid; idx < 36 * 16 * 4; idx += NTHR) {
+        const int quad = idx & 3, row = (idx >> 2) & 15, sb = idx >> 6;
+        const int qr = sb / 6, bpair = sb % 6;
+        const int y = qr >> 1, z = (qr & 1) ? (y == 2 ? 1 : 2) : (y == 0 ? 1 : 0);
+        const int ry = bpair >> 1, rz = (bpair & 1) ? (ry == 2 ? 1 : 2) : (ry == 0 ? 1 : 0);
+        const double *src = g.voe + (((long long)it.tpos[y] * g.no + it.tpos[z]) * g.nup + it.blk[ry] * TB + row) * g.nup
+                            + it.blk[rz] * TB + quad * 4;
+        double *d = V + sb * VBLK + row * VSTR + quad * 4;
+        d[0] = src[0]; d[1] = src[1]; d[2] = src[2]; d[3] = src[3];
+    }
+    if (tid < 144) {          // T[(y*3+n)*16+l] = T1(tuple[y], blk[n]*16+l)
+        const int l = tid & 15, yn = tid >> 4, y = yn / 3, n = yn % 3;
+        const int a = it.blk[n] * TB + l;
+        T[tid] = a < g.nu ? ga.t1[it.tpos[y] + (long long)g.no * a] : 0.0;
+    }
+    for (int idx = tid; idx < 36 * 256; idx += NTHR) D[idx] = 0.0;
+    __syncthreads();
+
+    const double dijk = g.eh[it.tpos[0]] + g.eh[it.tpos[1]] + g.eh[it.tpos[2]];
+    const double wt = tkind == 1 ? 1.0 : 0.5;
+    const double wgt = wt / (double)mult;
+    const int cl = lane & 15;
+    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
+    const int cg = C * TB + cl;
+    double *wb = ga.wbar + (long long)tl * nb * nb * nb * 4096;
+    double esum = 0.0;
+    double va[4][3], vb[BPW][3], vc[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) va[i][0] = va[i][1] = va[i][2] = 0.0;
+#pragma unroll
+    for (int i = 0; i < BPW; ++i) vb[i][0] = vb[i][1] = vb[i][2] = 0.0;
+    vc[0] = vc[1] = vc[2] = 0.0;
+
+
+    // pass 1: energy and v1 partials, as the forward epilogue
+    static_for<4>([&](auto VV) __attribute__((always_inline)) {
+        constexpr int v = decltype(VV)::value;
+        const int al = Mfma<double>::row(lane, v);
+        const int ag = A * TB + al;
+        static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
+            constexpr int bb = decltype(BB)::value;
+            const int bl = wave * BPW + bb;
+            const int bg = B * TB + bl;
+            double w[6];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
+            const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
+            if (ag < g.nu && bg < g.nu && cg < g.nu) {
+                const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
+                const double sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3] + w[4] * w[4] + w[5] * w[5];
+                esum += (6.0 * sq + 2.0 * ev * ev + 2.0 * od * od - 8.0 * ev * od) * inv;
+            }
+            const int el[3] = {al, bl, cl};
+            static_for<6>([&](auto S) __attribute__((always_inline)) {
+                constexpr int s = decltype(S)::value;
+                constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
+                constexpr int bpair = pair_index(r1, r2);
+                static_for<6>([&](auto PP) __attribute__((always_inline)) {
+                    constexpr int p = decltype(PP)::value;
+                    constexpr int q0 = wsrc_comp(p, s, 0), q1 = wsrc_comp(p, s, 2);
+                    constexpr int q2 = wsrc_comp(p, s, 1), q3 = wsrc_comp(p, s, 3);
+                    constexpr int qr = pair_index(PERM[p][1], PERM[p][2]);
+                    const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
+                    const double cv = z * inv * V[(qr * 6 + bpair) * VBLK + el[r1] * VSTR + el[r2]];
+                    if constexpr (r0 == 0) va[v][PERM[p][0]] += cv;
+                    else if constexpr (r0 == 1) vb[bb][PERM[p][0]] += cv;
+                    else vc[PERM[p][0]] += cv;
+                });
+            });
+        });
+    });
+
+    // ---- energy and v1 partials (as the forward kernel)
+    double *rw = red[wave];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double s = group16_sum(va[v][p]);
+            if ((lane & 15) == 0) rw[Mfma<double>::row(lane, v) * 3 + p] = s;
+        }
+#pragma unroll
+    for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double s = wave_sum(vb[bb][p]);
+            if (lane == 0) rw[48 + bb * 3 + p] = s;
+        }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        double s = vc[p];
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (lane < 16) rw[54 + lane * 3 + p] = s;
+    }
+    {
+        const double s = wave_sum(esum);
+        if (lane == 0) rw[102] = s;
+    }
+    __syncthreads();
+    double *vp = g.v1_part + item * 144;
+    if (tid < 48) {
+        double s = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][tid];
+        vp[tid] = s * wgt;
+    } else if (tid < 96) {
+        const int r = tid - 48, row = r / 3, p = r % 3;
+        vp[48 + r] = red[row / BPW][48 + (row % BPW) * 3 + p] * wgt;
+    } else if (tid < 144) {
+        const int r = tid - 96;
+        double s = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][54 + r];
+        vp[96 + r] = s * wgt;
+    } else if (tid == 144) {
+        double s = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][102];
+        g.e_part[item] = s * wgt;
+    }
+
+    __syncthreads();
+
+    // pass 2: dVOE partials, dVOE(x_r1, x_r2, t[P1], t[P2]) += Z inv T1(t[P0], x_r0), summed
+    // over the item's block of the contracted axis r0
+    static_for<4>([&](auto VV) __attribute__((always_inline)) {
+        constexpr int v = decltype(VV)::value;
+        const int al = Mfma<double>::row(lane, v);
+        const int ag = A * TB + al;
+        double dB[2][6];      // b-axis contracted: summed over batches, then waves
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) dB[s][p] = 0.0;
+        static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
+            constexpr int bb = decltype(BB)::value;
+            const int bl = wave * BPW + bb;
+            const int bg = B * TB + bl;
+            double w[6];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
+            const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
+            const int el[3] = {al, bl, cl};
+            static_for<6>([&](auto S) __attribute__((always_inline)) {
+                constexpr int s = decltype(S)::value;
+                constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
+                static_for<6>([&](auto PP) __attribute__((always_inline)) {
+                    constexpr int p = decltype(PP)::value;
+                    constexpr int q0 = wsrc_comp(p, s, 0), q1 = wsrc_comp(p, s, 2);
+                    constexpr int q2 = wsrc_comp(p, s, 1), q3 = wsrc_comp(p, s, 3);
+                    const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
+                    const double dv = z * inv * T[(PERM[p][0] * 3 + r0) * 16 + el[r0]];
+                    if constexpr (s < 2) {
+                        // a-axis: sum over the four row groups, then over v by the same lane
+                        // (s = 0 -> target (b, c); s = 1 -> (c, b))
+                        double r = dv;
+                        r += __shfl_xor(r, 16, 64);
+                        r += __shfl_xor(r, 32, 64);
+                        if (lane < 16) D[(s * 6 + p) * 256 + el[r1] * 16 + el[r2]] += r;
+                    } else if constexpr (s < 4) {
+                        dB[s - 2][p] += dv;
+                    } else {
+                        // c-axis: sum over the 16 lanes of equal row (s = 4 -> (a,b); 5 -> (b,a))
+                        double r = dv;
+                        r += __shfl_xor(r, 1, 64);
+                        r += __shfl_xor(r, 2, 64);
+                        r += __shfl_xor(r, 4, 64);
+                        r += __shfl_xor(r, 8, 64);
+                        if (cl == 0) D[(s * 6 + p) * 256 + el[r1] * 16 + el[r2]] = r;
+                    }
+                });
+            });
+        });
+        // b-axis: add the wave partials in wave order (deterministic)
+        for (int w2 = 0; w2 < NWAVE; ++w2) {
+            if (wave == w2) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int p = 0; p < 6; ++p) {
+                        // s = 2: (1,0,2) -> target (a, c); s = 3: (1,2,0) -> target (c, a)
+                        const int idx = s == 0 ? al * 16 + cl : cl * 16 + al;
+                        D[((s + 2) * 6 + p) * 256 + idx] += dB[s][p];
+                    }
+            }
+            __syncthreads();
+        }
+    });
+
+    // pass 3: Wbar at the six placements x^s = (x[PERM[s][0]], x[PERM[s][1]], x[PERM[s][2]])
+    static_for<4>([&](auto VV) __attribute__((always_inline)) {
+        constexpr int v = decltype(VV)::value;
+        const int al = Mfma<double>::row(lane, v);
+        const int ag = A * TB + al;
+        static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
+            constexpr int bb = decltype(BB)::value;
+            const int bl = wave * BPW + bb;
+            const int bg = B * TB + bl;
+            const int el[3] = {al, bl, cl};
+            double wets[6] = {0, 0, 0, 0, 0, 0};   // T1.VOE part of Wbar
+            static_for<6>([&](auto S) __attribute__((always_inline)) {
+                constexpr int s = decltype(S)::value;
+                constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
+                constexpr int bpair = pair_index(r1, r2);
+                static_for<6>([&](auto PP) __attribute__((always_inline)) {
+                    constexpr int p = decltype(PP)::value;
+                    constexpr int qr = pair_index(PERM[p][1], PERM[p][2]);
+                    const double pr = T[(PERM[p][0] * 3 + r0) * 16 + el[r0]] *
+                                      V[(qr * 6 + bpair) * VBLK + el[r1] * VSTR + el[r2]];
+                    static_for<6>([&](auto SS) __attribute__((always_inline)) {
+                        constexpr int s2 = decltype(SS)::value;
+                        static_for<4>([&](auto M) __attribute__((always_inline)) {
+                            constexpr int m = decltype(M)::value;
+                            if constexpr (sn_index(s2, p, m) == s) wets[s2] += ZC[m] * pr;
+                        });
+                    });
+                });
+            });
+            double w[6];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
+            const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
+            const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
+            const bool diag = ag == bg && bg == cg;
+            static_for<6>([&](auto S) __attribute__((always_inline)) {
+                constexpr int s = decltype(S)::value;
+                constexpr bool even = (s == 0 || s == 3 || s == 4);
+                const double ws = w[perm_inverse(s)];
+                const double same = even ? ev : od, other = even ? od : ev;
+                const double val = diag ? 0.0 : wt * inv * ((12.0 * ws + 4.0 * same - 8.0 * other) + 2.0 * wets[s]);
+                const int e0 = el[PERM[s][0]], e1 = el[PERM[s][1]], e2 = el[PERM[s][2]];
+                const int X = it.blk[PERM[s][0]], Y = it.blk[PERM[s][1]], Z = it.blk[PERM[s][2]];
+                // with equal blocks several placements cover the same block triple, from other
+                // threads and with other rounding: only the first placement writes it
+                bool first = true;
+#pragma unroll
+                for (int s2 = 0; s2 < s; ++s2)
+                    if (it.blk[PERM[s2][0]] == X && it.blk[PERM[s2][1]] == Y && it.blk[PERM[s2][2]] == Z) first = false;
+                if (first) wb[(((long long)X * nb + Y) * nb + Z) * 4096 + (e0 * 16 + e1) * 16 + e2] = val;
+            });
+        });
+    });
+
+    // ---- dVOE partials -> slots[tl][p][X][Y][Z]; placements that land on the same block
+    // triple (equal blocks) are summed here
+    const double f2 = 2.0 * wgt;
+    double *sl = ga.slots + (long long)tl * 6 * nb * nb * nb * 256;
+    for (int idx = tid; idx < 36 * 256; idx += NTHR) {
+        const int sp = idx >> 8, e = idx & 255;
+        const int s = sp / 6, p = sp % 6;
+        const int X = it.blk[PERM[s][1]], Y = it.blk[PERM[s][2]], Z = it.blk[PERM[s][0]];
+        bool first = true;
+        for (int s2 = 0; s2 < s; ++s2)
+            if (it.blk[PERM[s2][1]] == X && it.blk[PERM[s2][2]] == Y && it.blk[PERM[s2][0]] == Z) first = false;
+        if (!first) continue;
+        double sum = 0.0;
+        for (int s2 = s; s2 < 6; ++s2)
+            if (it.blk[PERM[s2][1]] == X && it.blk[PERM[s2][2]] == Y && it.blk[PERM[s2][0]] == Z)
+                sum += D[(s2 * 6 + p) * 256 + e];
+        sl[((((long long)p * nb + X) * nb + Y) * nb + Z) * 256 + e] = sum * f2;
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[NWAVE][112];
+    __shared__ long long next_item;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const TriplesArgs &g = ga.f;
+    const long long nlocal = (long long)ga.tcount * g.nblk;
+    for (;;) {
+        if (tid == 0) next_item = (long long)atomicAdd(g.counter, 1u);
+        __syncthreads();
+        const long long li = next_item;
+        __syncthreads();
+        if (li >= nlocal) break;
+        const int tl = __builtin_amdgcn_readfirstlane((int)(li / g.nblk));
+        const int q = __builtin_amdgcn_readfirstlane((int)(li % g.nblk));
+        const int t = ga.t0 + tl;
+        const long long item = (long long)t * g.nblk + q;
+        int tkind, mult;
+        const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
+        d4 acc[6][BPW];
+        if (tkind == 2) {
+            item_w<M_IJJ, double>(g, lds, it, acc, wave, lane);
+            adj_epilogue<M_IJJ>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
+        } else if (tkind == 3) {
+            item_w<M_IIJ, double>(g, lds, it, acc, wave, lane);
+            adj_epilogue<M_IIJ>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
+        } else if (mult == 1) {
+            item_w<M_FULL, double>(g, lds, it, acc, wave, lane);
+            adj_epilogue<M_FULL>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
+        } else if (mult == 6) {
+            item_w<M_AAA, double>(g, lds, it, acc, wave, lane);
+            adj_epilogue<M_AAA>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
+        } else if (it.blk[0] == it.blk[1]) {
+            item_w<M_AAC, double>(g, lds, it, acc, wave, lane);
+            adj_epilogue<M_AAC>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
+        } else {
+            item_w<M_ACC, double>(g, lds, it, acc, wave, lane);
+            adj_epilogue<M_ACC>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// back-contraction GEMMs (FP64 MFMA, LDS-staged, double buffered)
+// ---------------------------------------------------------------------------------
+namespace {
+constexpr int GCH = 128;                  // K columns per work group (8 MFMA tiles)
+constexpr int GCT = GCH / 16;
+constexpr int BS_STR = 17;                // padded row of the staged BX / AX slices
+}  // namespace
+
+struct GemmArgs {
+    const double *wbar;     // batch Wbar cubes
+    const double *axr, *axc, *bx;
+    double *gax;            // [no][nup][nup][kp]
+    double *gbx;            // [no][no][nup][kp]
+    const int *tuples;
+    int t0;
+    const int *ulist;       // gax: active x;  gbx: active y*no+z
+    const int *uoff;        // CSR offsets into ent, per ulist entry
+    const int *ent;         // (tl, sigma) pairs
+    int no, nup, nb, kp, nchunk, gt, ngg;
+};
+
+// gAX[x](R0 16 x R1 16 rows, K chunk) += sum over entries (t,sigma) with t[xp] = x and over
+// the gamma blocks G of Wbar_t(e_u = a, e_v = b, e_wf = g) BX[t[yp],t[zp]](g, K).
+// Wave w owns rows b in {2w, 2w+1}, all 8 K tiles of the chunk.
+__global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
+{
+    extern __shared__ double lds[];
+    double *WS = lds;                     // [2][WB_BLK]
+    double *BS = lds + 2 * WB_BLK;        // [2][GCH * BS_STR]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int u = blockIdx.x;
+    const int ch = u % a.nchunk; u /= a.nchunk;
+    const int R1 = u % a.nb; u /= a.nb;
+    const int R0 = u % a.nb; u /= a.nb;
+    const int ui = u;
+    const int x = a.ulist[ui];
+    const int e_beg = a.uoff[ui], e_end = a.uoff[ui + 1];
+    const int K0 = ch * GCH;
+    const long long cube = (long long)a.nb * a.nb * a.nb * 4096;
+    const int nsteps = (e_end - e_beg) * a.nb;
+
+    d4 acc[BPW][GCT];
+#pragma unroll
+    for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+        for (int ct = 0; ct < GCT; ++ct) acc[bb][ct] = d4{0, 0, 0, 0};
+
+    // staging registers: Wbar block 4096 = 512 x 8, BX slice 16 x 128 = 512 x 4
+    double rw[8], rb[4];
+    auto fetch = [&](int step) {
+        const int e = e_beg + step / a.nb, G = step % a.nb;
+        const int tl = a.ent[2 * e], sg = a.ent[2 * e + 1];
+        const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
+        const int tpos[3] = {tp.x, tp.y, tp.z};
+        const Sigma S = SIG[sg];
+        int bc[3];
+        bc[S.u] = R0; bc[S.v] = R1; bc[S.wf] = G;
+        const double *wsrc = a.wbar + tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double2 v = reinterpret_cast<const double2 *>(wsrc)[tid + k * NTHR];
+            rw[2 * k] = v.x; rw[2 * k + 1] = v.y;
+        }
+        const double *bsrc = a.bx + (((long long)tpos[S.yp] * a.no + tpos[S.zp]) * a.nb + G) * (16LL * a.kp);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int idx = 2 * (tid + k * NTHR);           // element of the 128 x 16 slice
+            const int Kl = idx >> 4;
+            double2 v = {0.0, 0.0};
+            if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(bsrc + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
+            rb[2 * k] = v.x; rb[2 * k + 1] = v.y;
+        }
+    };
+    auto stash = [&](int buf) {
+        double *W = WS + buf * WB_BLK;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int idx = 2 * (tid + k * NTHR);           // (a,b,c) with c even
+            const int aa = idx >> 8, b = (idx >> 4) & 15, c = idx & 15;
+            W[aa * WB_STR_A + b * WB_STR_B + c] = rw[2 * k];
+            W[aa * WB_STR_A + b * WB_STR_B + c + 1] = rw[2 * k + 1];
+        }
+        double *Bs = BS + buf * GCH * BS_STR;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int idx = 2 * (tid + k * NTHR);
+            const int Kl = idx >> 4, gl = idx & 15;
+            Bs[Kl * BS_STR + gl] = rb[2 * k];
+            Bs[Kl * BS_STR + gl + 1] = rb[2 * k + 1];
+        }
+    };
+
+    if (nsteps > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    for (int step = 0; step < nsteps; ++step) {
+        const int buf = step & 1;
+        if (step + 1 < nsteps) fetch(step + 1);
+        const int e = e_beg + step / a.nb;
+        const Sigma S = SIG[a.ent[2 * e + 1]];
+        const int str[3] = {WB_STR_A, WB_STR_B, 1};
+        const int su = str[S.u], sv = str[S.v], sw = str[S.wf];
+        const double *W = WS + buf * WB_BLK;
+        const double *Bs = BS + buf * GCH * BS_STR;
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) {
+            const int g = 4 * kq + (lane >> 4);
+            double af[BPW];
+#pragma unroll
+            for (int bb = 0; bb < BPW; ++bb) af[bb] = W[(lane & 15) * su + (wave * BPW + bb) * sv + g * sw];
+#pragma unroll
+            for (int ct = 0; ct < GCT; ++ct) {
+                const double bf = Bs[(ct * 16 + (lane & 15)) * BS_STR + g];
+#pragma unroll
+                for (int bb = 0; bb < BPW; ++bb) acc[bb][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bb], bf, acc[bb][ct], 0, 0, 0);
+            }
+        }
+        if (step + 1 < nsteps) stash(buf ^ 1);
+        __syncthreads();
+    }
+    // read-add-write of the output tile: rows a = row(lane,v), b = 2*wave+bb; column K
+#pragma unroll
+    for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+        for (int ct = 0; ct < GCT; ++ct) {
+            const int K = K0 + ct * 16 + (lane & 15);
+            if (K >= a.kp) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int al = Mfma<double>::row(lane, v);
+                double *o = a.gax + (((long long)x * a.nup + R0 * TB + al) * a.nup + R1 * TB + wave * BPW + bb) * a.kp + K;
+                *o += acc[bb][ct][v];
+            }
+        }
+}
+
+// gBX[y,z](gamma tiles of group gg, K chunk) += sum over entries (t,sigma) with
+// (t[yp],t[zp]) = (y,z), over panels (R0,R1) and the 256 (a,b) of a panel of
+// Wbar_t(e_u = a, e_v = b, e_wf = g) AX[t[xp]](a,b,K).  A panel is walked in 16 slices of
+// fixed e_sa (sa = axis 0 when u or v is 0, else axis 1), each slice staged in LDS:
+// Wbar (16 x gamma) and AX (16 x 128 K).  Wave w owns K tile w and all gamma tiles.
+__global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
+{
+    extern __shared__ double lds[];
+    const int gstr = a.gt * 16 + 1;       // padded row of the staged Wbar slice
+    double *WS = lds;                     // [2][16 * gstr]
+    double *AS = lds + 2 * 16 * gstr;     // [2][GCH * BS_STR]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int u = blockIdx.x;
+    const int gg = u % a.ngg; u /= a.ngg;
+    const int ch = u % a.nchunk; u /= a.nchunk;
+    const int ui = u;
+    const int yz = a.ulist[ui];
+    const int y = yz / a.no, z = yz % a.no;
+    const int e_beg = a.uoff[ui], e_end = a.uoff[ui + 1];
+    const int K0 = ch * GCH;
+    const int G0 = gg * a.gt;
+    const int ngt = (G0 + a.gt <= a.nb) ? a.gt : a.nb - G0;
+    const long long cube = (long long)a.nb * a.nb * a.nb * 4096;
+    const int per_entry = a.nb * a.nb * 16;
+    const int nsteps = (e_end - e_beg) * per_entry;
+    const int nw = ngt * 256;             // Wbar slice elements
+
+    d4 acc[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) acc[i] = d4{0, 0, 0, 0};
+
+    double rw[7], ra[4];
+    auto fetch = [&](int step) {
+        const int e = e_beg + step / per_entry;
+        int r = step % per_entry;
+        const int sidx = r & 15; r >>= 4;
+        const int R1 = r % a.nb, R0 = r / a.nb;
+        const int tl = a.ent[2 * e], sg = a.ent[2 * e + 1];
+        const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
+        const int tpos[3] = {tp.x, tp.y, tp.z};
+        const Sigma S = SIG[sg];
+        const int sa = (S.u == 0 || S.v == 0) ? 0 : 1;
+        const int oa = (sa == S.u) ? S.v : S.u;
+        // Wbar slice: element (o, g) = Wbar(e_sa = sidx, e_oa = o, e_wf = g) of block G0 + g/16
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int idx = tid + k * NTHR;
+            double v = 0.0;
+            if (idx < nw) {
+                const int gt_l = idx >> 8, o = (idx >> 4) & 15, gl = idx & 15;
+                int bc[3], ee[3];
+                bc[S.u] = R0; bc[S.v] = R1; bc[S.wf] = G0 + gt_l;
+                ee[sa] = sidx; ee[oa] = o; ee[S.wf] = gl;
+                v = a.wbar[tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096 + (ee[0] * 16 + ee[1]) * 16 + ee[2]];
+            }
+            rw[k] = v;
+        }
+        // AX slice: row layout (fixed alpha = e_u) or column layout (fixed beta = e_v)
+        const double *src = (sa == S.u ? a.axr : a.axc) +
+                            ((((long long)tpos[S.xp] * a.nb + R0) * a.nb + R1) * 16 + sidx) * (16LL * a.kp);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int idx = 2 * (tid + k * NTHR);
+            const int Kl = idx >> 4;
+            double2 v = {0.0, 0.0};
+            if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(src + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
+            ra[2 * k] = v.x; ra[2 * k + 1] = v.y;
+        }
+    };
+    auto stash = [&](int buf) {
+        double *W = WS + buf * 16 * gstr;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int idx = tid + k * NTHR;
+            if (idx < nw) {
+                const int gt_l = idx >> 8, o = (idx >> 4) & 15, gl = idx & 15;
+                W[o * gstr + gt_l * 16 + gl] = rw[k];
+            }
+        }
+        double *As = AS + buf * GCH * BS_STR;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int idx = 2 * (tid + k * NTHR);
+            const int Kl = idx >> 4, o = idx & 15;
+            As[Kl * BS_STR + o] = ra[2 * k];
+            As[Kl * BS_STR + o + 1] = ra[2 * k + 1];
+        }
+    };
+
+    if (nsteps > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    for (int step = 0; step < nsteps; ++step) {
+        const int buf = step & 1;
+        if (step + 1 < nsteps) fetch(step + 1);
+        const double *W = WS + buf * 16 * gstr;
+        const double *As = AS + buf * GCH * BS_STR;
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) {
+            const int o = 4 * kq + (lane >> 4);
+            const double bf = As[(wave * 16 + (lane & 15)) * BS_STR + o];
+#pragma unroll
+            for (int i = 0; i < 14; ++i)
+                if (i < ngt) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(W[o * gstr + i * 16 + (lane & 15)], bf, acc[i], 0, 0, 0);
+        }
+        if (step + 1 < nsteps) stash(buf ^ 1);
+        __syncthreads();
+    }
+    const int K = K0 + wave * 16 + (lane & 15);
+    if (K < a.kp) {
+#pragma unroll
+        for (int i = 0; i < 14; ++i) {
+            if (i >= ngt) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gamma = (G0 + i) * TB + Mfma<double>::row(lane, v);
+                double *o = a.gbx + (((long long)y * a.no + z) * a.nup + gamma) * a.kp + K;
+                *o += acc[i][v];
+            }
+        }
+    }
+}
+
+// dvoe[q][r][X*16+i][Y*16+j] += sum over entries (tl, p) of pair (q,r) and blocks Z of
+// slots[tl][p][X][Y][Z][i*16+j]
+__global__ void dvoe_reduce_kernel(const double *__restrict__ slots, double *__restrict__ dvoe,
+                                   const int *__restrict__ plist, const int *__restrict__ poff,
+                                   const int *__restrict__ ent, int np, int no, int nup, int nb)
+{
+    const long long per = (long long)nup * nup;
+    const long long total = (long long)np * per;
+    const long long tstride = 6LL * nb * nb * nb * 256;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int pi = (int)(idx / per);
+        const int rem = (int)(idx % per);
+        const int xr = rem / nup, yc = rem % nup;
+        const int X = xr >> 4, Y = yc >> 4, e = (xr & 15) * 16 + (yc & 15);
+        double s = 0.0;
+        for (int k = poff[pi]; k < poff[pi + 1]; ++k) {
+            const double *sl = slots + ent[2 * k] * tstride + ((((long long)ent[2 * k + 1] * nb + X) * nb + Y) * nb) * 256 + e;
+            for (int Z = 0; Z < nb; ++Z) s += sl[Z * 256];
+        }
+        dvoe[(long long)plist[pi] * per + rem] += s;
+    }
+}
+
+// gAX / gBX / dVOE accumulators and v1 -> reference layouts
+//   dVVVO(K,b,a,x) = gAX[x][a][b][K]                    (K < nu)
+//   dT2(a,b,i,j)   = gAX[j][a][b][nu+i] + gBX[i][j][a][b]
+//   dVM(l,g,z,y)   = -gBX[y][z][g][nu+l]
+//   dVOE(x,y,q,r)  = dvoe[q][r][x][y]
+//   dT1(o,a)       = 2 v1(a,o)
+__global__ void unpack_grad_kernel(const double *__restrict__ gax, const double *__restrict__ gbx,
+                                   const double *__restrict__ dvoep, const double *__restrict__ v1,
+                                   double *__restrict__ dvvvo, double *__restrict__ dt2, double *__restrict__ dvm,
+                                   double *__restrict__ dvoe, double *__restrict__ dt1, int no, int nu, int nup, int kp)
+{
+    const long long n_vvvo = (long long)nu * nu * nu * no;
+    const long long n_t2 = (long long)nu * nu * no * no;
+    const long long n_vm = (long long)no * nu * no * no;
+    const long long n_t1 = (long long)no * nu;
+    const long long total = n_vvvo + 2 * n_t2 + n_vm + n_t1;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long r = idx;
+        if (r < n_vvvo) {
+            if (dvvvo) {
+                const int K = r % nu; long long q = r / nu;
+                const int b = q % nu; q /= nu;
+                const int a = q % nu; const int x = (int)(q / nu);
+                dvvvo[r] = gax[(((long long)x * nup + a) * nup + b) * kp + K];
+            }
+            continue;
+        }
+        r -= n_vvvo;
+        if (r < n_t2) {
+            if (dt2) {
+                const int a = r % nu; long long q = r / nu;
+                const int b = q % nu; q /= nu;
+                const int i = q % no; const int j = (int)(q / no);
+                dt2[r] = gax[(((long long)j * nup + a) * nup + b) * kp + nu + i] +
+                         gbx[(((long long)i * no + j) * nup + a) * kp + b];
+            }
+            continue;
+        }
+        r -= n_t2;
+        if (r < n_vm) {
+            if (dvm) {
+                const int l = r % no; long long q = r / no;
+                const int gm = q % nu; q /= nu;
+                const int z = q % no; const int y = (int)(q / no);
+                dvm[r] = -gbx[(((long long)y * no + z) * nup + gm) * kp + nu + l];
+            }
+            continue;
+        }
+        r -= n_vm;
+        if (r < n_t2) {
+            if (dvoe) {
+                const int x = r % nu; long long q = r / nu;
+                const int y = q % nu; q /= nu;
+                const int qq = q % no; const int rr = (int)(q / no);
+                dvoe[r] = dvoep[(((long long)qq * no + rr) * nup + x) * nup + y];
+            }
+            continue;
+        }
+        r -= n_t2;
+        if (dt1) {
+            const int o = r % no, a = (int)(r / no);
+            dt1[r] = 2.0 * v1[a + (long long)nu * o];
+        }
+    }
+}
+
+extern "C" {
+
+int ccx_hip_adj_lds_bytes(void) { return (int)(sizeof(double) * ADJ_LDS); }
+
+int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const double *voep, const double *eh,
+                      const double *ep, const double *t1, const int *tuples, const int *blocks, double *e_part,
+                      double *v1_part, double *wbar, double *slots, unsigned int *counter, int no, int nu, int nup,
+                      int kp, int ntup, int nblk, int t0, int tcount, int grid, void *stream)
+{
+    static bool attr_set = false;
+    const size_t lds_bytes = sizeof(double) * ADJ_LDS;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)adj_item_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_bytes) != hipSuccess)
+            return 2;
+        attr_set = true;
+    }
+    AdjArgs ga;
+    TriplesArgs &g = ga.f;
+    g.axr = axr; g.axc = axc; g.bx = bx; g.voe = voep; g.eh = eh; g.ep = ep; g.tuples = tuples;
+    g.blocks = blocks; g.e_part = e_part; g.v1_part = v1_part; g.wout = nullptr; g.counter = counter;
+    g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
+    g.nitems = (long long)ntup * nblk;
+    ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.t0 = t0; ga.tcount = tcount;
+    const long long nlocal = (long long)tcount * nblk;
+    if (nlocal == 0) return 0;
+    if (grid <= 0) grid = 256;
+    if (grid > nlocal) grid = (int)nlocal;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemsetAsync(counter, 0, sizeof(unsigned int), s) != hipSuccess) return 1;
+    adj_item_kernel<<<grid, NTHR, lds_bytes, s>>>(ga);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_gax(const double *wbar, const double *bx, double *gax, const int *tuples, int t0, const int *xlist,
+                const int *xoff, const int *ent, int nx, int no, int nup, int kp, void *stream)
+{
+    static bool attr_set = false;
+    const size_t lds_bytes = sizeof(double) * (2 * WB_BLK + 2 * GCH * BS_STR);
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)gax_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_bytes) != hipSuccess)
+            return 2;
+        attr_set = true;
+    }
+    if (nx == 0) return 0;
+    GemmArgs a = {};
+    a.wbar = wbar; a.bx = bx; a.gax = gax; a.tuples = tuples; a.t0 = t0;
+    a.ulist = xlist; a.uoff = xoff; a.ent = ent;
+    a.no = no; a.nup = nup; a.nb = nup / TB; a.kp = kp; a.nchunk = (kp + GCH - 1) / GCH;
+    const long long grid = (long long)nx * a.nb * a.nb * a.nchunk;
+    gax_kernel<<<(unsigned)grid, NTHR, lds_bytes, (hipStream_t)stream>>>(a);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_gbx(const double *wbar, const double *axr, const double *axc, double *gbx, const int *tuples, int t0,
+                const int *plist, const int *poff, const int *ent, int npair, int no, int nup, int kp, void *stream)
+{
+    const int nb = nup / TB;
+    const int ngg = (nb + 13) / 14;
+    const int gt = (nb + ngg - 1) / ngg;
+    static bool attr_set = false;
+    const size_t lds_max = sizeof(double) * (2 * 16 * (14 * 16 + 1) + 2 * GCH * BS_STR);
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)gbx_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_max) != hipSuccess)
+            return 2;
+        attr_set = true;
+    }
+    if (npair == 0) return 0;
+    GemmArgs a = {};
+    a.wbar = wbar; a.axr = axr; a.axc = axc; a.gbx = gbx; a.tuples = tuples; a.t0 = t0;
+    a.ulist = plist; a.uoff = poff; a.ent = ent;
+    a.no = no; a.nup = nup; a.nb = nb; a.kp = kp; a.nchunk = (kp + GCH - 1) / GCH; a.gt = gt; a.ngg = ngg;
+    const size_t lds_bytes = sizeof(double) * (2 * 16 * (gt * 16 + 1) + 2 * GCH * BS_STR);
+    const long long grid = (long long)npair * a.nchunk * ngg;
+    gbx_kernel<<<(unsigned)grid, NTHR, lds_bytes, (hipStream_t)stream>>>(a);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_dvoe_reduce(const double *slots, double *dvoep, const int *plist, const int *poff, const int *ent,
+                        int np, int no, int nup, void *stream)
+{
+    if (np == 0) return 0;
+    const long long total = (long long)np * nup * nup;
+    long long grid = (total + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    dvoe_reduce_kernel<<<(unsigned)grid, 256, 0, (hipStream_t)stream>>>(slots, dvoep, plist, poff, ent, np, no, nup,
+                                                                       nup / TB);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_unpack_grad(const double *gax, const double *gbx, const double *dvoep, const double *v1, double *dvvvo,
+                        double *dt2, double *dvm, double *dvoe, double *dt1, int no, int nu, int nup, int kp,
+                        void *stream)
+{
+    const long long total = (long long)nu * nu * nu * no + 2LL * nu * nu * no * no + (long long)no * nu * no * no +
+                            (long long)no * nu;
+    long long grid = (total + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    unpack_grad_kernel<<<(unsigned)grid, 256, 0, (hipStream_t)stream>>>(gax, gbx, dvoep, v1, dvvvo, dt2, dvm, dvoe, dt1,
+                                                                       no, nu, nup, kp);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_zero(void *p, size_t bytes, void *stream)
+{
+    if (bytes == 0) return 0;
+    return hipMemsetAsync(p, 0, bytes, (hipStream_t)stream) == hipSuccess ? 0 : 1;
+}
+
+}  // extern "C"

@@ -399,6 +399,167 @@ int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
     return CCX_OK;
 }
 
+/* ------------------------------------------------------------------ adjoint */
+
+/* sigma-terms of W (x position, AX index axes u,v, BX pair positions y,z, BX axis wf):
+ * W_t(e) = sum_sigma sum_K AX[t[xp]](e_u,e_v,K) BX[t[yp],t[zp]](e_wf,K); same table as
+ * the device code */
+static const int SIGMA_H[6][6] = {{0, 0, 1, 1, 2, 2}, {1, 1, 0, 0, 2, 2}, {2, 2, 1, 1, 0, 0},
+                                  {0, 0, 2, 2, 1, 1}, {1, 1, 2, 2, 0, 0}, {2, 2, 0, 0, 1, 1}};
+static const int PERM_H[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+/* CSR lists of (tl, code) entries grouped by key, keys in increasing order, entries in
+ * (tl, code) order: list[u], off[u..u+1] for the nonzero keys only.  Returns #keys. */
+static int build_csr(int nkeys, int n, const int *key, const int *tl, const int *code, int *list, int *off,
+                     int *ent, int *count)
+{
+    memset(count, 0, sizeof(int) * (size_t)nkeys);
+    for (int q = 0; q < n; ++q) count[key[q]]++;
+    int nu_ = 0, pos = 0;
+    for (int k = 0; k < nkeys; ++k)
+        if (count[k]) {
+            list[nu_] = k;
+            off[nu_] = pos;
+            pos += count[k];
+            count[k] = off[nu_];    /* running insert position */
+            ++nu_;
+        }
+    off[nu_] = pos;
+    for (int q = 0; q < n; ++q) {   /* stable: entries keep (tl, code) order */
+        const int at = count[key[q]]++;
+        ent[2 * at] = tl[q];
+        ent[2 * at + 1] = code[q];
+    }
+    return nu_;
+}
+
+int ccx_gradient(ccx_ctx *c, double *ets, double *etd, double *dt1, double *dt2, double *dvm, double *dvoe,
+                 double *dvvvo, int on_device)
+{
+    if (!c) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    const int64_t no = c->no, nu = c->nu, nup = c->nup, nb = c->nb, kp = c->kp;
+    const int64_t ntup = c->ntup, nblk = c->nblk;
+    int rc = CCX_OK;
+    double *d_gax = NULL, *d_gbx = NULL, *d_dvoep = NULL, *d_wbar = NULL, *d_slots = NULL;
+    double *d_out[5] = {NULL, NULL, NULL, NULL, NULL};
+    int *d_lists = NULL, *h_lists = NULL, *h_tmp = NULL;
+    double host_out[2] = {0.0, 0.0};
+    /* batch of tuples: Wbar cubes + dVOE slots within a memory budget (CCX_ADJ_BATCH_MB) */
+    const double per_tuple = 8.0 * (double)(nup * nup * nup) + 8.0 * 6.0 * (double)(nb * nb * nb) * 256.0;
+    double budget = 16384.0;
+    const char *env = getenv("CCX_ADJ_BATCH_MB");
+    if (env && atof(env) > 0) budget = atof(env);
+    int64_t tb = (int64_t)(budget * 1048576.0 / per_tuple);
+    if (tb < 1) tb = 1;
+    if (tb > ntup) tb = ntup > 0 ? ntup : 1;
+    const int64_t nent = 6 * tb;                   /* entries per list kind per batch */
+    const int64_t nkey = no * no;
+    /* host list scratch: key/tl/code arrays (3 x nent), count (nkey), and three CSR sets */
+    const int64_t lsz = 3 * (nkey + (nkey + 1) + 2 * nent);
+    h_tmp = (int *)malloc(sizeof(int) * (size_t)(3 * nent + nkey));
+    h_lists = (int *)malloc(sizeof(int) * (size_t)lsz);
+    if (!h_tmp || !h_lists) { rc = CCX_ERR_ALLOC; goto fail; }
+    DALLOC(d_gax, no * nup * nup * kp);
+    DALLOC(d_gbx, no * no * nup * kp);
+    DALLOC(d_dvoep, no * no * nup * nup);
+    DALLOC(d_wbar, tb * nup * nup * nup);
+    DALLOC(d_slots, tb * 6 * nb * nb * nb * 256);
+    DALLOC(d_lists, lsz);
+    if (ccx_hip_zero(d_gax, sizeof(double) * (size_t)(no * nup * nup * kp), c->stream) != 0 ||
+        ccx_hip_zero(d_gbx, sizeof(double) * (size_t)(no * no * nup * kp), c->stream) != 0 ||
+        ccx_hip_zero(d_dvoep, sizeof(double) * (size_t)(no * no * nup * nup), c->stream) != 0) {
+        rc = CCX_ERR_DEVICE;
+        goto fail;
+    }
+    if (ccx_hip_event_record(c->ev[0], c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    for (int64_t t0 = 0; t0 < ntup; t0 += tb) {
+        const int tcount = (int)(t0 + tb <= ntup ? tb : ntup - t0);
+        int *key = h_tmp, *tlv = h_tmp + nent, *code = h_tmp + 2 * nent, *cnt = h_tmp + 3 * nent;
+        /* three CSR sets laid out back to back: [list nkey][off nkey+1][ent 2 nent] */
+        const int64_t set = nkey + (nkey + 1) + 2 * nent;
+        int nkeys[3];
+        for (int kind = 0; kind < 3; ++kind) {
+            int n = 0;
+            for (int tl = 0; tl < tcount; ++tl) {
+                const int *tp = c->h_tuples + 4 * (t0 + tl);
+                for (int q = 0; q < 6; ++q) {
+                    if (kind == 0) key[n] = tp[SIGMA_H[q][0]];                               /* gAX: x */
+                    else if (kind == 1) key[n] = tp[SIGMA_H[q][3]] * (int)no + tp[SIGMA_H[q][4]]; /* gBX: (y,z) */
+                    else key[n] = tp[PERM_H[q][1]] * (int)no + tp[PERM_H[q][2]];           /* dVOE: (q,r) */
+                    tlv[n] = tl;
+                    code[n] = q;
+                    ++n;
+                }
+            }
+            int *base = h_lists + kind * set;
+            nkeys[kind] = build_csr((int)nkey, n, key, tlv, code, base, base + nkey, base + nkey + (nkey + 1), cnt);
+        }
+        if (ccx_hip_memcpy_h2d(d_lists, h_lists, sizeof(int) * (size_t)lsz, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+        const int *dl[3][3];
+        for (int kind = 0; kind < 3; ++kind) {
+            dl[kind][0] = d_lists + kind * set;
+            dl[kind][1] = dl[kind][0] + nkey;
+            dl[kind][2] = dl[kind][1] + (nkey + 1);
+        }
+        if (ccx_hip_adj_items(c->d_axr, c->d_axc, c->d_bx, c->d_voep, c->d_eh, c->d_ep, c->d_t1, c->d_tuples,
+                              c->d_blocks, c->d_epart, c->d_v1part, d_wbar, d_slots, c->d_counter, (int)no, (int)nu,
+                              (int)nup, (int)kp, (int)ntup, (int)nblk, (int)t0, tcount,
+                              grid_size(c, (int64_t)tcount * nblk), c->stream) != 0 ||
+            ccx_hip_gax(d_wbar, c->d_bx, d_gax, c->d_tuples, (int)t0, dl[0][0], dl[0][1], dl[0][2], nkeys[0], (int)no,
+                        (int)nup, (int)kp, c->stream) != 0 ||
+            ccx_hip_gbx(d_wbar, c->d_axr, c->d_axc, d_gbx, c->d_tuples, (int)t0, dl[1][0], dl[1][1], dl[1][2],
+                        nkeys[1], (int)no, (int)nup, (int)kp, c->stream) != 0 ||
+            ccx_hip_dvoe_reduce(d_slots, d_dvoep, dl[2][0], dl[2][1], dl[2][2], nkeys[2], (int)no, (int)nup,
+                                c->stream) != 0) {
+            rc = CCX_ERR_DEVICE;
+            goto fail;
+        }
+    }
+    if (ccx_hip_finalize(c->d_epart, c->d_v1part, c->d_tuples, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,
+                         (int)no, (int)nu, (int)ntup, (int)nblk, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    {
+        double *outs[5] = {dvvvo, dt2, dvm, dvoe, dt1};
+        const int64_t sizes[5] = {nu * nu * nu * no, nu * nu * no * no, no * nu * no * no, nu * nu * no * no, no * nu};
+        double *dst[5];
+        for (int q = 0; q < 5; ++q) {
+            dst[q] = NULL;
+            if (!outs[q]) continue;
+            if (on_device) dst[q] = outs[q];
+            else { DALLOC(d_out[q], sizes[q]); dst[q] = d_out[q]; }
+        }
+        if (ccx_hip_unpack_grad(d_gax, d_gbx, d_dvoep, c->d_v1, dst[0], dst[1], dst[2], dst[3], dst[4], (int)no,
+                                (int)nu, (int)nup, (int)kp, c->stream) != 0 ||
+            ccx_hip_event_record(c->ev[3], c->stream) != 0 ||
+            ccx_hip_memcpy_d2h(host_out, c->d_out, sizeof(double) * 2, c->stream) != 0) {
+            rc = CCX_ERR_DEVICE;
+            goto fail;
+        }
+        for (int q = 0; q < 5; ++q)
+            if (outs[q] && !on_device &&
+                ccx_hip_memcpy_d2h(outs[q], d_out[q], sizeof(double) * (size_t)sizes[q], c->stream) != 0) {
+                rc = CCX_ERR_DEVICE;
+                goto fail;
+            }
+    }
+    {
+        float t_ms = 0.0f;
+        if (ccx_hip_event_ms(c->ev[0], c->ev[3], &t_ms) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+        c->kernel_ms = t_ms;
+        c->total_ms = t_ms;
+    }
+    if (etd) *etd = host_out[0];
+    if (ets) *ets = host_out[1];
+fail:
+    ccx_hip_sync(c->stream);
+    ccx_hip_free(d_gax); ccx_hip_free(d_gbx); ccx_hip_free(d_dvoep); ccx_hip_free(d_wbar); ccx_hip_free(d_slots);
+    ccx_hip_free(d_lists);
+    for (int q = 0; q < 5; ++q) ccx_hip_free(d_out[q]);
+    free(h_tmp);
+    free(h_lists);
+    return rc;
+}
+
 /* ------------------------------------------------------------------ legacy Fortran ABI */
 
 static int64_t g_no = 0, g_nu = 0;

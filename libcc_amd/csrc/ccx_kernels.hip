@@ -29,141 +29,9 @@
 //
 // FP64 MFMA v_mfma_f64_16x16x4_f64: lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // and holds D[l/16 + 4v][l%16], v = 0..3 (checked on the device).
-#include <hip/hip_runtime.h>
-#include <stdint.h>
-#include <type_traits>
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
+#include "ccx_item.h"
 
-namespace {
-
-constexpr int TB = 16;                 // virtual block edge
-constexpr int NWAVE = 8;               // waves per work group (2 per SIMD)
-constexpr int NTHR = 64 * NWAVE;
-constexpr int BPW = TB / NWAVE;        // batches (16x16 output tiles) per wave = 2
-constexpr int KS = 4;                  // K per MFMA step
-constexpr int CSTR_B = 17;             // padded strides of a 16^3 cube in LDS
-constexpr int CSTR_A = TB * CSTR_B;    // 272
-constexpr int CUBE = TB * CSTR_A;      // 4352 doubles
-constexpr int VSTR = 17;
-constexpr int VBLK = TB * VSTR;        // 272 doubles per staged VOE sub-block
-constexpr int LDS_DOUBLES = (3 * CUBE > 36 * VBLK) ? 3 * CUBE : 36 * VBLK;
-
-// orderings of the occupied tuple positions (0,1,2); even: 0,3,4
-__device__ constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-
-__host__ __device__ constexpr int pair_index(int y, int z) { return y * 2 + (z > y ? z - 1 : z); }
-
-__host__ __device__ constexpr int perm_index(int a, int b, int c)
-{
-    return a == 0 ? (b == 1 ? 0 : 1) : a == 1 ? (b == 0 ? 2 : 3) : (b == 0 ? 4 : 5);
-}
-
-// Term tables per phase (0: gamma=b, 1: gamma=c, 2: gamma=a) and ordering p:
-// operand slot (orientation*3 + tuple position of x) and BX slot (ordered position pair).
-struct Term { int s1, b1, s2, b2; };
-
-__host__ __device__ constexpr Term term(int phase, int p)
-{
-    const int p1 = PERM[p][0], p2 = PERM[p][1], p3 = PERM[p][2];
-    if (phase == 0)       // AX[p1](a,c) BX[p3,p2](b) ; AX[p3](c,a) BX[p1,p2](b)
-        return {p1, pair_index(p3, p2), 3 + p3, pair_index(p1, p2)};
-    else if (phase == 1)  // AX[p1](a,b) BX[p2,p3](c) ; AX[p2](b,a) BX[p1,p3](c)
-        return {p1, pair_index(p2, p3), 3 + p2, pair_index(p1, p3)};
-    else                  // AX[p2](b,c) BX[p3,p1](a) ; AX[p3](c,b) BX[p2,p1](a)
-        return {p2, pair_index(p3, p1), 3 + p3, pair_index(p2, p1)};
-}
-
-// w_P at the virtual position pi(e) equals w_Q at e, with Q_n = P_{pi^-1(n)}.
-__host__ __device__ constexpr int wsrc(int p, const int *pi)
-{
-    int inv[3] = {0, 0, 0};
-    inv[pi[0]] = 0;
-    inv[pi[1]] = 1;
-    inv[pi[2]] = 2;
-    return perm_index(PERM[p][inv[0]], PERM[p][inv[1]], PERM[p][inv[2]]);
-}
-
-__host__ __device__ constexpr int wsrc_comp(int p, int s, int t)
-{
-    int pi[3] = {PERM[s][PERM[t][0]], PERM[s][PERM[t][1]], PERM[s][PERM[t][2]]};
-    return wsrc(p, pi);
-}
-
-// Item modes: which orderings are accumulated.
-enum Mode { M_FULL = 0, M_AAC = 1, M_ACC = 2, M_AAA = 3, M_IJJ = 4, M_IIJ = 5 };
-
-__host__ __device__ constexpr int mode_mask(int mode)
-{
-    return mode == M_FULL ? 0x3f : mode == M_AAC ? 0x0b /*0,1,3*/ : mode == M_ACC ? 0x15 /*0,2,4*/
-         : mode == M_AAA ? 0x01 : mode == M_IJJ ? 0x0d /*0,2,3*/ : 0x13 /*0,1,4*/;
-}
-
-// operand slots a phase needs for the computed orderings
-__host__ __device__ constexpr int slot_mask(int phase, int mode)
-{
-    int m = 0;
-    for (int p = 0; p < 6; ++p)
-        if (mode_mask(mode) & (1 << p)) m |= (1 << term(phase, p).s1) | (1 << term(phase, p).s2);
-    return m;
-}
-
-__host__ __device__ constexpr int bx_mask(int phase, int mode)
-{
-    int m = 0;
-    for (int p = 0; p < 6; ++p)
-        if (mode_mask(mode) & (1 << p)) m |= (1 << term(phase, p).b1) | (1 << term(phase, p).b2);
-    return m;
-}
-
-template <int N, int I = 0, class F>
-__device__ __forceinline__ void static_for(F &&f)
-{
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>{});
-        static_for<N, I + 1>(f);
-    }
-}
-
-// The two MFMA forms: operand type T, accumulator tile type, and the row a lane/item
-// of a 16x16 output tile holds (column = lane & 15 for both).
-template <class T> struct Mfma;
-template <> struct Mfma<double> {
-    typedef d4 acc_t;
-    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-template <> struct Mfma<float> {
-    typedef f4 acc_t;
-    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
-}  // namespace
-
-struct TriplesArgs {
-    const void *axr;       // [no][nb][nb][16 alpha][kp/4][4][16 beta]  (row fragments), type T
-    const void *axc;       // [no][nb][nb][16 beta][kp/4][4][16 alpha]  (column fragments), type T
-    const void *bx;        // [no][no][nb][kp/4][4][16 gamma], type T
-    const double *voe;     // [no][no][nup][nup], voe[q][r][x][y] = VOE(x,y,q,r)
-    const double *eh;      // [no]
-    const double *ep;      // [nup], padded entries finite
-    const int *tuples;     // [ntup][4]: i, j, k, kind (1 distinct, 2 = (i,j,j), 3 = (i,i,j))
-    const int *blocks;     // [nblk][4]: A, B, C, multiplicity (1, 2 or 6)
-    double *e_part;        // [nitems]
-    double *v1_part;       // [nitems][3][16][3]
-    double *wout;          // W_ijk cube (nu^3, column-major) of tuple 0, or null
-    unsigned int *counter; // work queue head (zeroed before the launch)
-    int no, nu, nup, nb, kp, ntup, nblk;
-    long long nitems;
-};
 
 // ---------------------------------------------------------------------------------
 // packing kernels (once per problem)
@@ -264,249 +132,12 @@ __global__ void unpack_vvvo_kernel(const double *__restrict__ packed, double *__
 // ---------------------------------------------------------------------------------
 namespace {
 
-struct ItemCtx {
-    int tpos[3];        // occupied indices of the tuple positions
-    int blk[3];         // A, B, C
-    int ys[6], zs[6];   // BX slot -> ordered occupied pair
-};
-
-// Fragment base pointers (wave-uniform) of one phase for this wave's two batches; a
-// fragment of K step kb is base[kb*64 + lane].
-template <class T>
-struct PhasePtrs {
-    const T *p[6][BPW];   // operand slot, batch
-    const T *x[6];        // BX slot
-};
-
-template <int PHASE, class T>
-__device__ __forceinline__ void phase_ptrs(const TriplesArgs &g, const ItemCtx &it, int wave, PhasePtrs<T> &pp)
-{
-    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
-    // orientation 0 / 1 panels (R0 = alpha block, R1 = beta block) and column-read flags
-    int r0[2], r1[2];
-    bool col[2];
-    if (PHASE == 0) { r0[0] = A; r1[0] = C; col[0] = true;  r0[1] = C; r1[1] = A; col[1] = false; }
-    else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
-    else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
-    const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
-    const long long per_panel = 256LL * g.kp;
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const int o = s / 3, xp = s % 3;
-        const T *base = static_cast<const T *>(col[o] ? g.axc : g.axr) +
-                        (((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o]) * per_panel;
-#pragma unroll
-        for (int bb = 0; bb < BPW; ++bb) pp.p[s][bb] = base + (long long)(wave * BPW + bb) * (16LL * g.kp);
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-        pp.x[q] = static_cast<const T *>(g.bx) + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (16LL * g.kp);
-}
-
-template <class T>
-struct Frags {
-    T p[6][BPW];
-    T x[6];
-};
-
-template <int PHASE, int MODE, class T>
-__device__ __forceinline__ void load_frags(const PhasePtrs<T> &pp, int kb, int lane, Frags<T> &f)
-{
-    constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
-    const int off = kb * 64 + lane;
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-        if (SM & (1 << s))
-#pragma unroll
-            for (int bb = 0; bb < BPW; ++bb) f.p[s][bb] = pp.p[s][bb][off];
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-        if (XM & (1 << q)) f.x[q] = pp.x[q][off];
-}
-
-template <int PHASE, int MODE, class T>
-__device__ __forceinline__ void mfma_step(const Frags<T> &f, typename Mfma<T>::acc_t (&acc)[6][BPW])
-{
-    constexpr int MM = mode_mask(MODE);
-    // first term of every computed ordering, then the second, so no MFMA waits on its predecessor
-#pragma unroll
-    for (int bb = 0; bb < BPW; ++bb)
-        static_for<6>([&](auto P) {
-            constexpr int p = decltype(P)::value;
-            if constexpr ((MM >> p) & 1) {
-                constexpr Term t = term(PHASE, p);
-                if constexpr (PHASE == 2)
-                    acc[p][bb] = Mfma<T>::mac(f.x[t.b1], f.p[t.s1][bb], acc[p][bb]);
-                else
-                    acc[p][bb] = Mfma<T>::mac(f.p[t.s1][bb], f.x[t.b1], acc[p][bb]);
-            }
-        });
-#pragma unroll
-    for (int bb = 0; bb < BPW; ++bb)
-        static_for<6>([&](auto P) {
-            constexpr int p = decltype(P)::value;
-            if constexpr ((MM >> p) & 1) {
-                constexpr Term t = term(PHASE, p);
-                if constexpr (PHASE == 2)
-                    acc[p][bb] = Mfma<T>::mac(f.x[t.b2], f.p[t.s2][bb], acc[p][bb]);
-                else
-                    acc[p][bb] = Mfma<T>::mac(f.p[t.s2][bb], f.x[t.b2], acc[p][bb]);
-            }
-        });
-}
-
-// One K sweep.  Register double buffering: fragments of step kb+1 load while step kb runs.
-template <int PHASE, int MODE, class T>
-__device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &it,
-                                          typename Mfma<T>::acc_t (&acc)[6][BPW], int wave, int lane)
-{
-    PhasePtrs<T> pp;
-    phase_ptrs<PHASE, T>(g, it, wave, pp);
-    const int nkb = g.kp / KS;
-    Frags<T> f0, f1;
-    load_frags<PHASE, MODE, T>(pp, 0, lane, f0);
-    int kb = 0;
-    for (; kb + 2 <= nkb; kb += 2) {
-        load_frags<PHASE, MODE, T>(pp, kb + 1, lane, f1);
-        mfma_step<PHASE, MODE, T>(f0, acc);
-        if (kb + 2 < nkb) load_frags<PHASE, MODE, T>(pp, kb + 2, lane, f0);
-        mfma_step<PHASE, MODE, T>(f1, acc);
-    }
-    if (kb < nkb) mfma_step<PHASE, MODE, T>(f0, acc);
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ double group16_sum(double v)
-{
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Element (a,b,c) of a 16^3 cube in LDS.
-__device__ __forceinline__ int cidx(int a, int b, int c) { return a * CSTR_A + b * CSTR_B + c; }
-
-// Layout L (after phase 0): wave owns batches b = wave*BPW+bb; lane/v give row a = (lane>>4)+4v,
-// column c = lane&15.  Layout Lb (phase 0): batches c, rows a, columns b.
-template <int MODE, class T>
-__device__ __forceinline__ void transpose_lb_to_l(double *lds, typename Mfma<T>::acc_t (&acc)[6][BPW], int wave,
-                                                  int lane)
-{
-    constexpr int MM = mode_mask(MODE);
-    constexpr int NC = __builtin_popcount(MM);
-    // up to 3 cubes per round
-    static_for<2>([&](auto R) {
-        constexpr int round = decltype(R)::value;
-        if constexpr (round * 3 < NC) {
-            int slot = 0;
-            static_for<6>([&](auto P) {
-                constexpr int p = decltype(P)::value;
-                constexpr int rank = __builtin_popcount(MM & ((1 << p) - 1));
-                if constexpr (((MM >> p) & 1) && rank / 3 == round) {
-                    double *cube = lds + (rank % 3) * CUBE;
-#pragma unroll
-                    for (int bb = 0; bb < BPW; ++bb)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            cube[cidx(Mfma<T>::row(lane, v), lane & 15, wave * BPW + bb)] = acc[p][bb][v];
-                }
-            });
-            (void)slot;
-            __syncthreads();
-            static_for<6>([&](auto P) {
-                constexpr int p = decltype(P)::value;
-                constexpr int rank = __builtin_popcount(MM & ((1 << p) - 1));
-                if constexpr (((MM >> p) & 1) && rank / 3 == round) {
-                    const double *cube = lds + (rank % 3) * CUBE;
-#pragma unroll
-                    for (int bb = 0; bb < BPW; ++bb)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            acc[p][bb][v] = cube[cidx(Mfma<T>::row(lane, v), wave * BPW + bb, lane & 15)];
-                }
-            });
-            __syncthreads();
-        }
-    });
-}
-
-// Fill the orderings a mode did not accumulate (layout L).
-template <int MODE, class T>
-__device__ __forceinline__ void complete_orderings(double *lds, typename Mfma<T>::acc_t (&acc)[6][BPW], int wave,
-                                                   int lane)
-{
-    if constexpr (MODE == M_FULL) {
-        return;
-    } else if constexpr (MODE == M_IJJ) {   // (i,j,j): w1 = w0, w4 = w2, w5 = w3
-#pragma unroll
-        for (int bb = 0; bb < BPW; ++bb) { acc[1][bb] = acc[0][bb]; acc[4][bb] = acc[2][bb]; acc[5][bb] = acc[3][bb]; }
-    } else if constexpr (MODE == M_IIJ) {   // (i,i,j): w2 = w0, w3 = w1, w5 = w4
-#pragma unroll
-        for (int bb = 0; bb < BPW; ++bb) { acc[2][bb] = acc[0][bb]; acc[3][bb] = acc[1][bb]; acc[5][bb] = acc[4][bb]; }
-    } else {
-        // Equal blocks: w_P(e) = w_Q(pi.e) read from the cube of Q in LDS.
-        constexpr int MM = mode_mask(MODE);
-        static_for<6>([&](auto P) {
-            constexpr int p = decltype(P)::value;
-            constexpr int rank = __builtin_popcount(MM & ((1 << p) - 1));
-            if constexpr ((MM >> p) & 1) {
-                double *cube = lds + rank * CUBE;
-#pragma unroll
-                for (int bb = 0; bb < BPW; ++bb)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        cube[cidx(Mfma<T>::row(lane, v), wave * BPW + bb, lane & 15)] = acc[p][bb][v];
-            }
-        });
-        __syncthreads();
-#pragma unroll
-        for (int bb = 0; bb < BPW; ++bb)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int e[3] = {Mfma<T>::row(lane, v), wave * BPW + bb, lane & 15};
-                if constexpr (MODE == M_AAC) {         // w2(a,b,c)=w0(b,a,c), w4=w1(b,a,c), w5=w3(b,a,c)
-                    acc[2][bb][v] = lds[0 * CUBE + cidx(e[1], e[0], e[2])];
-                    acc[4][bb][v] = lds[1 * CUBE + cidx(e[1], e[0], e[2])];
-                    acc[5][bb][v] = lds[2 * CUBE + cidx(e[1], e[0], e[2])];
-                } else if constexpr (MODE == M_ACC) {  // w1=w0(a,c,b), w3=w2(a,c,b), w5=w4(a,c,b)
-                    acc[1][bb][v] = lds[0 * CUBE + cidx(e[0], e[2], e[1])];
-                    acc[3][bb][v] = lds[1 * CUBE + cidx(e[0], e[2], e[1])];
-                    acc[5][bb][v] = lds[2 * CUBE + cidx(e[0], e[2], e[1])];
-                } else {                               // A=B=C: w_P(e) = w_0(P^-1 . e)
-                    static_for<5>([&](auto Q) {
-                        constexpr int p = decltype(Q)::value + 1;
-                        constexpr int i0 = PERM[p][0] == 0 ? 0 : PERM[p][1] == 0 ? 1 : 2;
-                        constexpr int i1 = PERM[p][0] == 1 ? 0 : PERM[p][1] == 1 ? 1 : 2;
-                        constexpr int i2 = PERM[p][0] == 2 ? 0 : PERM[p][1] == 2 ? 1 : 2;
-                        acc[p][bb][v] = lds[cidx(e[i0], e[i1], e[i2])];
-                    });
-                }
-            }
-        __syncthreads();
-    }
-}
-
 template <int MODE, class T>
 __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
                         int tkind, int mult, int wave, int lane, int tid)
 {
     typename Mfma<T>::acc_t acc[6][BPW];
-#pragma unroll
-    for (int p = 0; p < 6; ++p)
-#pragma unroll
-        for (int bb = 0; bb < BPW; ++bb) acc[p][bb] = typename Mfma<T>::acc_t{0, 0, 0, 0};
-
-    run_phase<0, MODE, T>(g, it, acc, wave, lane);
-    transpose_lb_to_l<MODE, T>(lds, acc, wave, lane);
-    run_phase<1, MODE, T>(g, it, acc, wave, lane);
-    run_phase<2, MODE, T>(g, it, acc, wave, lane);
-    complete_orderings<MODE, T>(lds, acc, wave, lane);
+    item_w<MODE, T>(g, lds, it, acc, wave, lane);
 
     const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
     // ---- stage the 36 VOE sub-blocks: [ordered tuple-position pair][ordered block pair]
@@ -555,11 +186,11 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
             const double sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3] + w[4] * w[4] + w[5] * w[5];
             esum += (6.0 * sq + 2.0 * ev * ev + 2.0 * od * od - 8.0 * ev * od) * inv;
             const int el[3] = {al, bl, cl};
-            static_for<6>([&](auto S) {
+            static_for<6>([&](auto S) __attribute__((always_inline)) {
                 constexpr int s = decltype(S)::value;
                 constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
                 constexpr int bpair = pair_index(r1, r2);
-                static_for<6>([&](auto PP) {
+                static_for<6>([&](auto PP) __attribute__((always_inline)) {
                     constexpr int p = decltype(PP)::value;
                     constexpr int q0 = wsrc_comp(p, s, 0);   // sigma
                     constexpr int q1 = wsrc_comp(p, s, 2);   // sigma o (1,0,2)
@@ -576,7 +207,7 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
             });
             if (g.wout != nullptr) {
                 const int xg[3] = {ag, bg, cg};
-                static_for<6>([&](auto S) {
+                static_for<6>([&](auto S) __attribute__((always_inline)) {
                     constexpr int s = decltype(S)::value;
                     constexpr int q0 = wsrc_comp(0, s, 0);
                     g.wout[xg[PERM[s][0]] + (long long)g.nu * (xg[PERM[s][1]] + (long long)g.nu * xg[PERM[s][2]])] = w[q0];
@@ -656,26 +287,13 @@ __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
         if (item >= g.nitems) break;
         const int t = __builtin_amdgcn_readfirstlane((int)(item / g.nblk));
         const int q = __builtin_amdgcn_readfirstlane((int)(item % g.nblk));
-        int4 tp = reinterpret_cast<const int4 *>(g.tuples)[t];
-        int4 bp = reinterpret_cast<const int4 *>(g.blocks)[q];
-        tp.x = __builtin_amdgcn_readfirstlane(tp.x); tp.y = __builtin_amdgcn_readfirstlane(tp.y);
-        tp.z = __builtin_amdgcn_readfirstlane(tp.z); tp.w = __builtin_amdgcn_readfirstlane(tp.w);
-        bp.x = __builtin_amdgcn_readfirstlane(bp.x); bp.y = __builtin_amdgcn_readfirstlane(bp.y);
-        bp.z = __builtin_amdgcn_readfirstlane(bp.z); bp.w = __builtin_amdgcn_readfirstlane(bp.w);
-        ItemCtx it;
-        it.tpos[0] = tp.x; it.tpos[1] = tp.y; it.tpos[2] = tp.z;
-        it.blk[0] = bp.x; it.blk[1] = bp.y; it.blk[2] = bp.z;
-#pragma unroll
-        for (int y = 0; y < 3; ++y)
-#pragma unroll
-            for (int z = 0; z < 3; ++z)
-                if (y != z) { it.ys[pair_index(y, z)] = it.tpos[y]; it.zs[pair_index(y, z)] = it.tpos[z]; }
-        const int tkind = tp.w, mult = bp.w;
+        int tkind, mult;
+        const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
         if (tkind == 2) do_item<M_IJJ, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
         else if (tkind == 3) do_item<M_IIJ, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
         else if (mult == 1) do_item<M_FULL, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
         else if (mult == 6) do_item<M_AAA, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (bp.x == bp.y) do_item<M_AAC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
         else do_item<M_ACC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
     }
 }

@@ -81,6 +81,29 @@ class TriplesContext:
                                     ctypes.c_void_p(v1.ctypes.data), 0), "ccx_energy")
         return {"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value, "v1": v1}
 
+    def gradient(self, device: bool = False) -> dict:
+        """E(T) and dE/d(t1, t2, vm, voe, vvvo) in the reference layouts (FP64, ccx_gradient).
+
+        With ``device=True`` the gradients are torch float64 tensors on the current GPU,
+        written in place by the library; otherwise Fortran-ordered numpy arrays."""
+        no, nu = self.no, self.nu
+        shapes = {"t1": (no, nu), "t2": (nu, nu, no, no), "vm": (no, nu, no, no),
+                  "voe": (nu, nu, no, no), "vvvo": (nu, nu, nu, no)}
+        if device:
+            import torch
+            # Fortran order = reversed-shape contiguous tensor, transposed back
+            out = {k: torch.empty(tuple(reversed(v)), dtype=torch.float64, device="cuda").permute(
+                *reversed(range(len(v)))) for k, v in shapes.items()}
+            ptrs = {k: ctypes.c_void_p(t.data_ptr()) for k, t in out.items()}
+        else:
+            out = {k: np.zeros(v, order="F") for k, v in shapes.items()}
+            ptrs = {k: ctypes.c_void_p(a.ctypes.data) for k, a in out.items()}
+        ets, etd = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ccx_gradient(self._ctx, ctypes.byref(ets), ctypes.byref(etd), ptrs["t1"], ptrs["t2"],
+                                  ptrs["vm"], ptrs["voe"], ptrs["vvvo"], 1 if device else 0), "ccx_gradient")
+        out.update({"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value})
+        return out
+
     def tuple_w(self, i: int, j: int, k: int) -> np.ndarray:
         """W_ijk cube for a 1-based tuple (no W(a,a,a)=0 step)."""
         w = np.zeros((self.nu, self.nu, self.nu), order="F")

@@ -167,5 +167,5 @@ def grad_dense(inp: dict) -> dict:
     e = etd + ets
     e.backward()
     out = {k: np.asfortranarray(leaves[k].grad.numpy()) for k in leaves}
-    out["e"] = float(e)
+    out["e"] = float(e.detach())
     return out

@@ -57,3 +57,15 @@ def test_q_form_equals_reference_summand():
     ev, od = w[0] + w[3] + w[4], w[1] + w[2] + w[5]
     q = 6 * np.dot(w, w) + 2 * ev * ev + 2 * od * od - 8 * ev * od
     assert abs(q - want) < 1e-12 * abs(want)
+
+
+def test_adjoint_epilogue_tables_match_per_tuple_adjoint():
+    """The item-epilogue tables of the adjoint kernel (Wbar at six placements, the T1.VOE
+    products, dVOE partials) against the per-tuple adjoint, with padded 4-blocks."""
+    import adjoint_kernel_model
+    import adjoint_model
+    inp = synth.make_inputs(3, 6, 7)
+    want = adjoint_model.model_grad(inp)
+    got = adjoint_kernel_model.model_adjoint(inp, tb=4)
+    for k in ("vvvo", "t2", "vm", "voe"):
+        np.testing.assert_allclose(got[k], want[k], rtol=0, atol=1e-14 * np.abs(want[k]).max())

@@ -68,15 +68,22 @@ struct AdjArgs {
     const double *t1;       // T1(no, nu), column-major
     double *wbar;           // [tcount][nb][nb][nb][16][16][16]
     double *slots;          // [tcount][6][nb][nb][nb][16][16]  dVOE partials
+    double *wstage;         // [grid][6][BPW][4][NTHR]: the item's W, one slab per work group
     int t0, tcount;         // tuples of this batch
 };
 
 namespace {
 
 // Epilogue of one adjoint work item; acc holds w_p in layout L.
-template <int MODE>
+// The item's W is read back per point from the work group's staging slab ws (written by
+// the same thread), so that no accumulator registers stay live through the epilogue.
+__device__ __forceinline__ double staged_w(const double *ws, int p, int bb, int v, int tid)
+{
+    return ws[((p * BPW + bb) * 4 + v) * NTHR + tid];
+}
+
 __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112], const ItemCtx &it,
-                             long long item, int tl, int tkind, int mult, d4 (&acc)[6][BPW], int wave, int lane,
+                             long long item, int tl, int tkind, int mult, const double *ws, int wave, int lane,
                              int tid)
 {
     const TriplesArgs &g = ga.f;
@@ -99,6 +106,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
         T[tid] = a < g.nu ? ga.t1[it.tpos[y] + (long long)g.no * a] : 0.0;
     }
     for (int idx = tid; idx < 36 * 256; idx += NTHR) D[idx] = 0.0;
+    for (int idx = lane; idx < 112; idx += 64) red[wave][idx] = 0.0;
     __syncthreads();
 
     const double dijk = g.eh[it.tpos[0]] + g.eh[it.tpos[1]] + g.eh[it.tpos[2]];
@@ -108,18 +116,18 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
     const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
     const int cg = C * TB + cl;
     double *wb = ga.wbar + (long long)tl * nb * nb * nb * 4096;
+    double *rw = red[wave];
     double esum = 0.0;
-    double va[4][3], vb[BPW][3], vc[3];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) va[i][0] = va[i][1] = va[i][2] = 0.0;
-#pragma unroll
-    for (int i = 0; i < BPW; ++i) vb[i][0] = vb[i][1] = vb[i][2] = 0.0;
-    vc[0] = vc[1] = vc[2] = 0.0;
 
-
-    // pass 1: energy and v1 partials, as the forward epilogue
-    static_for<4>([&](auto VV) __attribute__((always_inline)) {
-        constexpr int v = decltype(VV)::value;
+    // The three passes run over this thread's 8 points (rows a = row(lane,v), batches
+    // b = wave*BPW+bb, column c = lane&15) with a run-time loop over v, so only one row
+    // of points is live at a time.
+    // pass 1: energy and v1 partials (the forward epilogue), reduced per point:
+    //   role a (rows al): over the 16 lanes of a row group  -> rw[al*3 + p]
+    //   role b (rows bl): over the wave                     -> rw[48 + bb*3 + p]
+    //   role c (rows cl): over the four row groups          -> rw[54 + cl*3 + p]
+#pragma unroll 1
+    for (int v = 0; v < 4; ++v) {
         const int al = Mfma<double>::row(lane, v);
         const int ag = A * TB + al;
         static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
@@ -128,7 +136,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
             const int bg = B * TB + bl;
             double w[6];
 #pragma unroll
-            for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
+            for (int p = 0; p < 6; ++p) w[p] = staged_w(ws, p, bb, v, tid);
             const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
             if (ag < g.nu && bg < g.nu && cg < g.nu) {
                 const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
@@ -136,6 +144,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
                 esum += (6.0 * sq + 2.0 * ev * ev + 2.0 * od * od - 8.0 * ev * od) * inv;
             }
             const int el[3] = {al, bl, cl};
+            double ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
             static_for<6>([&](auto S) __attribute__((always_inline)) {
                 constexpr int s = decltype(S)::value;
                 constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
@@ -147,36 +156,23 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
                     constexpr int qr = pair_index(PERM[p][1], PERM[p][2]);
                     const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
                     const double cv = z * inv * V[(qr * 6 + bpair) * VBLK + el[r1] * VSTR + el[r2]];
-                    if constexpr (r0 == 0) va[v][PERM[p][0]] += cv;
-                    else if constexpr (r0 == 1) vb[bb][PERM[p][0]] += cv;
-                    else vc[PERM[p][0]] += cv;
+                    if constexpr (r0 == 0) ra[PERM[p][0]] += cv;
+                    else if constexpr (r0 == 1) rb[PERM[p][0]] += cv;
+                    else rc[PERM[p][0]] += cv;
                 });
             });
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const double sa = group16_sum(ra[p]);
+                if (cl == 0) rw[al * 3 + p] += sa;
+                const double sb = wave_sum(rb[p]);
+                if (lane == 0) rw[48 + bb * 3 + p] += sb;
+                double sc = rc[p];
+                sc += __shfl_xor(sc, 16, 64);
+                sc += __shfl_xor(sc, 32, 64);
+                if (lane < 16) rw[54 + cl * 3 + p] += sc;
+            }
         });
-    });
-
-    // ---- energy and v1 partials (as the forward kernel)
-    double *rw = red[wave];
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            const double s = group16_sum(va[v][p]);
-            if ((lane & 15) == 0) rw[Mfma<double>::row(lane, v) * 3 + p] = s;
-        }
-#pragma unroll
-    for (int bb = 0; bb < BPW; ++bb)
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            const double s = wave_sum(vb[bb][p]);
-            if (lane == 0) rw[48 + bb * 3 + p] = s;
-        }
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-        double s = vc[p];
-        s += __shfl_xor(s, 16, 64);
-        s += __shfl_xor(s, 32, 64);
-        if (lane < 16) rw[54 + lane * 3 + p] = s;
     }
     {
         const double s = wave_sum(esum);
@@ -205,12 +201,10 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
         g.e_part[item] = s * wgt;
     }
 
-    __syncthreads();
-
     // pass 2: dVOE partials, dVOE(x_r1, x_r2, t[P1], t[P2]) += Z inv T1(t[P0], x_r0), summed
     // over the item's block of the contracted axis r0
-    static_for<4>([&](auto VV) __attribute__((always_inline)) {
-        constexpr int v = decltype(VV)::value;
+#pragma unroll 1
+    for (int v = 0; v < 4; ++v) {
         const int al = Mfma<double>::row(lane, v);
         const int ag = A * TB + al;
         double dB[2][6];      // b-axis contracted: summed over batches, then waves
@@ -224,7 +218,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
             const int bg = B * TB + bl;
             double w[6];
 #pragma unroll
-            for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
+            for (int p = 0; p < 6; ++p) w[p] = staged_w(ws, p, bb, v, tid);
             const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
             const int el[3] = {al, bl, cl};
             static_for<6>([&](auto S) __attribute__((always_inline)) {
@@ -271,11 +265,11 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
             }
             __syncthreads();
         }
-    });
+    }
 
     // pass 3: Wbar at the six placements x^s = (x[PERM[s][0]], x[PERM[s][1]], x[PERM[s][2]])
-    static_for<4>([&](auto VV) __attribute__((always_inline)) {
-        constexpr int v = decltype(VV)::value;
+#pragma unroll 1
+    for (int v = 0; v < 4; ++v) {
         const int al = Mfma<double>::row(lane, v);
         const int ag = A * TB + al;
         static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
@@ -304,16 +298,16 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
             });
             double w[6];
 #pragma unroll
-            for (int p = 0; p < 6; ++p) w[p] = acc[p][bb][v];
+            for (int p = 0; p < 6; ++p) w[p] = staged_w(ws, p, bb, v, tid);
             const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
             const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
             const bool diag = ag == bg && bg == cg;
             static_for<6>([&](auto S) __attribute__((always_inline)) {
                 constexpr int s = decltype(S)::value;
                 constexpr bool even = (s == 0 || s == 3 || s == 4);
-                const double ws = w[perm_inverse(s)];
+                const double ws_ = w[perm_inverse(s)];
                 const double same = even ? ev : od, other = even ? od : ev;
-                const double val = diag ? 0.0 : wt * inv * ((12.0 * ws + 4.0 * same - 8.0 * other) + 2.0 * wets[s]);
+                const double val = diag ? 0.0 : wt * inv * ((12.0 * ws_ + 4.0 * same - 8.0 * other) + 2.0 * wets[s]);
                 const int e0 = el[PERM[s][0]], e1 = el[PERM[s][1]], e2 = el[PERM[s][2]];
                 const int X = it.blk[PERM[s][0]], Y = it.blk[PERM[s][1]], Z = it.blk[PERM[s][2]];
                 // with equal blocks several placements cover the same block triple, from other
@@ -325,7 +319,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
                 if (first) wb[(((long long)X * nb + Y) * nb + Z) * 4096 + (e0 * 16 + e1) * 16 + e2] = val;
             });
         });
-    });
+    }
 
     // ---- dVOE partials -> slots[tl][p][X][Y][Z]; placements that land on the same block
     // triple (equal blocks) are summed here
@@ -372,25 +366,20 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
         int tkind, mult;
         const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
         d4 acc[6][BPW];
-        if (tkind == 2) {
-            item_w<M_IJJ, double>(g, lds, it, acc, wave, lane);
-            adj_epilogue<M_IJJ>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
-        } else if (tkind == 3) {
-            item_w<M_IIJ, double>(g, lds, it, acc, wave, lane);
-            adj_epilogue<M_IIJ>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
-        } else if (mult == 1) {
-            item_w<M_FULL, double>(g, lds, it, acc, wave, lane);
-            adj_epilogue<M_FULL>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
-        } else if (mult == 6) {
-            item_w<M_AAA, double>(g, lds, it, acc, wave, lane);
-            adj_epilogue<M_AAA>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
-        } else if (it.blk[0] == it.blk[1]) {
-            item_w<M_AAC, double>(g, lds, it, acc, wave, lane);
-            adj_epilogue<M_AAC>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
-        } else {
-            item_w<M_ACC, double>(g, lds, it, acc, wave, lane);
-            adj_epilogue<M_ACC>(ga, lds, red, it, item, tl, tkind, mult, acc, wave, lane, tid);
-        }
+        if (tkind == 2) item_w<M_IJJ, double>(g, lds, it, acc, wave, lane);
+        else if (tkind == 3) item_w<M_IIJ, double>(g, lds, it, acc, wave, lane);
+        else if (mult == 1) item_w<M_FULL, double>(g, lds, it, acc, wave, lane);
+        else if (mult == 6) item_w<M_AAA, double>(g, lds, it, acc, wave, lane);
+        else if (it.blk[0] == it.blk[1]) item_w<M_AAC, double>(g, lds, it, acc, wave, lane);
+        else item_w<M_ACC, double>(g, lds, it, acc, wave, lane);
+        double *ws = ga.wstage + (long long)blockIdx.x * (6 * BPW * 4 * NTHR);
+#pragma unroll
+        for (int p = 0; p < 6; ++p)
+#pragma unroll
+            for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) ws[((p * BPW + bb) * 4 + v) * NTHR + tid] = acc[p][bb][v];
+        adj_epilogue(ga, lds, red, it, item, tl, tkind, mult, ws, wave, lane, tid);
     }
 }
 
@@ -413,7 +402,8 @@ struct GemmArgs {
     const int *ulist;       // gax: active x;  gbx: active y*no+z
     const int *uoff;        // CSR offsets into ent, per ulist entry
     const int *ent;         // (tl, sigma) pairs
-    int no, nup, nb, kp, nchunk, gt, ngg;
+    double *part;           // gbx: per (entry, split, chunk, gamma group) partial tiles
+    int no, nup, nb, kp, nchunk, gt, ngg, nsplit;
 };
 
 // gAX[x](R0 16 x R1 16 rows, K chunk) += sum over entries (t,sigma) with t[xp] = x and over
@@ -534,15 +524,19 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
         }
 }
 
-// gBX[y,z](gamma tiles of group gg, K chunk) += sum over entries (t,sigma) with
-// (t[yp],t[zp]) = (y,z), over panels (R0,R1) and the 256 (a,b) of a panel of
-// Wbar_t(e_u = a, e_v = b, e_wf = g) AX[t[xp]](a,b,K).  A panel is walked in 16 slices of
-// fixed e_sa (sa = axis 0 when u or v is 0, else axis 1), each slice staged in LDS:
-// Wbar (16 x gamma) and AX (16 x 128 K).  Wave w owns K tile w and all gamma tiles.
+// Partial gBX tile of one entry (t,sigma) of pair (y,z) = (t[yp],t[zp]): gamma tiles of group
+// gg, K chunk ch, panels R0 in split rs (all R1), summed over the 256 (a,b) of a panel:
+//   part(g, K) = sum Wbar_t(e_u = a, e_v = b, e_wf = g) AX[t[xp]](a,b,K).
+// One work group per (entry, split, chunk, group), so pairs that collect many entries in a
+// batch spread over many work groups; gbx_reduce_kernel adds the partials of a pair in a
+// fixed order.  A panel is walked in 16 slices of fixed e_sa (sa = axis 0 when u or v is 0,
+// else axis 1), each slice staged in LDS: Wbar (16 x gamma) and AX (16 x 128 K).  Wave w
+// owns K tile w and all gamma tiles.
+template <int GT>
 __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
 {
     extern __shared__ double lds[];
-    const int gstr = a.gt * 16 + 1;       // padded row of the staged Wbar slice
+    constexpr int gstr = GT * 16 + 1;     // padded row of the staged Wbar slice
     double *WS = lds;                     // [2][16 * gstr]
     double *AS = lds + 2 * 16 * gstr;     // [2][GCH * BS_STR]
     const int tid = threadIdx.x, lane = tid & 63;
@@ -550,45 +544,52 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
     int u = blockIdx.x;
     const int gg = u % a.ngg; u /= a.ngg;
     const int ch = u % a.nchunk; u /= a.nchunk;
-    const int ui = u;
-    const int yz = a.ulist[ui];
-    const int y = yz / a.no, z = yz % a.no;
-    const int e_beg = a.uoff[ui], e_end = a.uoff[ui + 1];
+    const int rs = u % a.nsplit; u /= a.nsplit;
+    const int e = u;                      // entry (CSR position)
+    const int r0_beg = rs * a.nb / a.nsplit, r0_end = (rs + 1) * a.nb / a.nsplit;
     const int K0 = ch * GCH;
-    const int G0 = gg * a.gt;
-    const int ngt = (G0 + a.gt <= a.nb) ? a.gt : a.nb - G0;
+    const int G0 = gg * GT;
+    const int ngt = (G0 + GT <= a.nb) ? GT : a.nb - G0;   // tiles past ngt are zero
     const long long cube = (long long)a.nb * a.nb * a.nb * 4096;
-    const int per_entry = a.nb * a.nb * 16;
-    const int nsteps = (e_end - e_beg) * per_entry;
-    const int nw = ngt * 256;             // Wbar slice elements
+    const int nsteps = (r0_end - r0_beg) * a.nb * 16;
+    constexpr int nw = GT * 256;          // Wbar slice elements (zero past ngt tiles)
 
-    d4 acc[14];
+    d4 acc[GT];
 #pragma unroll
-    for (int i = 0; i < 14; ++i) acc[i] = d4{0, 0, 0, 0};
+    for (int i = 0; i < GT; ++i) acc[i] = d4{0, 0, 0, 0};
 
-    double rw[7], ra[4];
+    // Staging registers.  The Wbar slice is read in memory order: with the slice axis sa
+    // fixed, the block element offset is m>>4 along axis hi (1 if sa = 0, else 0) and m&15
+    // along axis 2, so a wave reads 128-byte runs whatever sigma is.
+    double2 rw[4];
+    double ra[4];
+    int st_tr;                            // 1: the slice's (hi, 2) axes are (o, g); 0: (g, o)
     auto fetch = [&](int step) {
-        const int e = e_beg + step / per_entry;
-        int r = step % per_entry;
+        int r = step;
         const int sidx = r & 15; r >>= 4;
-        const int R1 = r % a.nb, R0 = r / a.nb;
+        const int R1 = r % a.nb, R0 = r0_beg + r / a.nb;
         const int tl = a.ent[2 * e], sg = a.ent[2 * e + 1];
         const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
         const int tpos[3] = {tp.x, tp.y, tp.z};
         const Sigma S = SIG[sg];
         const int sa = (S.u == 0 || S.v == 0) ? 0 : 1;
+        const int hi = sa == 0 ? 1 : 0;
         const int oa = (sa == S.u) ? S.v : S.u;
-        // Wbar slice: element (o, g) = Wbar(e_sa = sidx, e_oa = o, e_wf = g) of block G0 + g/16
+        st_tr = (oa == hi) ? 1 : 0;
+        int bc[3];
+        bc[S.u] = R0; bc[S.v] = R1; bc[S.wf] = G0;
+        const long long gstep = S.wf == 0 ? (long long)a.nb * a.nb : S.wf == 1 ? a.nb : 1;
+        const double *wbase = a.wbar + tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096 +
+                              (sa == 0 ? sidx * 256 : sidx * 16);
+        const int hstr = sa == 0 ? 16 : 256;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            const int idx = tid + k * NTHR;
-            double v = 0.0;
-            if (idx < nw) {
-                const int gt_l = idx >> 8, o = (idx >> 4) & 15, gl = idx & 15;
-                int bc[3], ee[3];
-                bc[S.u] = R0; bc[S.v] = R1; bc[S.wf] = G0 + gt_l;
-                ee[sa] = sidx; ee[oa] = o; ee[S.wf] = gl;
-                v = a.wbar[tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096 + (ee[0] * 16 + ee[1]) * 16 + ee[2]];
+        for (int k = 0; k < 4; ++k) {
+            const int idx = tid + k * NTHR;               // double2 index in the slice
+            double2 v = {0.0, 0.0};
+            if (idx < nw / 2) {
+                const int gt_l = idx >> 7, m = (idx & 127) * 2;
+                if (gt_l < ngt)
+                    v = *reinterpret_cast<const double2 *>(wbase + gt_l * gstep * 4096 + (m >> 4) * hstr + (m & 15));
             }
             rw[k] = v;
         }
@@ -607,11 +608,18 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
     auto stash = [&](int buf) {
         double *W = WS + buf * 16 * gstr;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const int idx = tid + k * NTHR;
-            if (idx < nw) {
-                const int gt_l = idx >> 8, o = (idx >> 4) & 15, gl = idx & 15;
-                W[o * gstr + gt_l * 16 + gl] = rw[k];
+            if (idx < nw / 2) {
+                const int gt_l = idx >> 7, m = (idx & 127) * 2;
+                const int h = m >> 4, l = m & 15;     // l, l+1 along axis 2
+                if (st_tr) {                          // o = h, g = l, l+1
+                    W[h * gstr + gt_l * 16 + l] = rw[k].x;
+                    W[h * gstr + gt_l * 16 + l + 1] = rw[k].y;
+                } else {                              // o = l, l+1, g = h
+                    W[l * gstr + gt_l * 16 + h] = rw[k].x;
+                    W[(l + 1) * gstr + gt_l * 16 + h] = rw[k].y;
+                }
             }
         }
         double *As = AS + buf * GCH * BS_STR;
@@ -639,24 +647,41 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
             const int o = 4 * kq + (lane >> 4);
             const double bf = As[(wave * 16 + (lane & 15)) * BS_STR + o];
 #pragma unroll
-            for (int i = 0; i < 14; ++i)
-                if (i < ngt) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(W[o * gstr + i * 16 + (lane & 15)], bf, acc[i], 0, 0, 0);
+            for (int i = 0; i < GT; ++i)
+                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(W[o * gstr + i * 16 + (lane & 15)], bf, acc[i], 0, 0, 0);
         }
         if (step + 1 < nsteps) stash(buf ^ 1);
         __syncthreads();
     }
-    const int K = K0 + wave * 16 + (lane & 15);
-    if (K < a.kp) {
+    double *pt = a.part + (((long long)e * a.nsplit + rs) * a.nchunk + ch) * a.ngg * (GT * 16 * GCH) +
+                 (long long)gg * (GT * 16 * GCH);
 #pragma unroll
-        for (int i = 0; i < 14; ++i) {
-            if (i >= ngt) continue;
+    for (int i = 0; i < GT; ++i)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int gamma = (G0 + i) * TB + Mfma<double>::row(lane, v);
-                double *o = a.gbx + (((long long)y * a.no + z) * a.nup + gamma) * a.kp + K;
-                *o += acc[i][v];
-            }
-        }
+        for (int v = 0; v < 4; ++v)
+            pt[(i * 16 + Mfma<double>::row(lane, v)) * GCH + wave * 16 + (lane & 15)] = acc[i][v];
+}
+
+// gbx[y][z][g][K] += sum over the entries of pair (y,z) (CSR order) and the panel splits of
+// the partial tiles
+__global__ void gbx_reduce_kernel(const double *__restrict__ part, double *__restrict__ gbx,
+                                  const int *__restrict__ plist, const int *__restrict__ poff, int np, int nup,
+                                  int kp, int nchunk, int ngg, int gtr, int nsplit)
+{
+    const long long per = (long long)nup * kp;
+    const long long total = (long long)np * per;
+    const long long tile = (long long)gtr * GCH;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int pi = (int)(idx / per);
+        const int rem = (int)(idx % per);
+        const int g = rem / kp, K = rem % kp;
+        const int ch = K / GCH, kl = K % GCH, gg = g / gtr, gl = g % gtr;
+        double s = 0.0;
+        for (int e = poff[pi]; e < poff[pi + 1]; ++e)
+            for (int rs = 0; rs < nsplit; ++rs)
+                s += part[((((long long)e * nsplit + rs) * nchunk + ch) * ngg + gg) * tile + (long long)gl * GCH + kl];
+        gbx[(long long)plist[pi] * per + rem] += s;
     }
 }
 
@@ -751,14 +776,31 @@ __global__ void unpack_grad_kernel(const double *__restrict__ gax, const double 
     }
 }
 
+template <int GT>
+static int launch_gbx(const GemmArgs &a, long long grid, hipStream_t s)
+{
+    static bool attr_set = false;
+    const size_t lds_bytes = sizeof(double) * (2 * 16 * (GT * 16 + 1) + 2 * GCH * BS_STR);
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)gbx_kernel<GT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_bytes) != hipSuccess)
+            return 2;
+        attr_set = true;
+    }
+    gbx_kernel<GT><<<(unsigned)grid, NTHR, lds_bytes, s>>>(a);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 extern "C" {
 
 int ccx_hip_adj_lds_bytes(void) { return (int)(sizeof(double) * ADJ_LDS); }
 
+size_t ccx_hip_adj_stage_doubles(void) { return (size_t)6 * BPW * 4 * NTHR; }
+
 int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const double *voep, const double *eh,
                       const double *ep, const double *t1, const int *tuples, const int *blocks, double *e_part,
-                      double *v1_part, double *wbar, double *slots, unsigned int *counter, int no, int nu, int nup,
-                      int kp, int ntup, int nblk, int t0, int tcount, int grid, void *stream)
+                      double *v1_part, double *wbar, double *slots, double *wstage, unsigned int *counter, int no,
+                      int nu, int nup, int kp, int ntup, int nblk, int t0, int tcount, int grid, void *stream)
 {
     static bool attr_set = false;
     const size_t lds_bytes = sizeof(double) * ADJ_LDS;
@@ -774,7 +816,7 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     g.blocks = blocks; g.e_part = e_part; g.v1_part = v1_part; g.wout = nullptr; g.counter = counter;
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
-    ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.t0 = t0; ga.tcount = tcount;
+    ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.wstage = wstage; ga.t0 = t0; ga.tcount = tcount;
     const long long nlocal = (long long)tcount * nblk;
     if (nlocal == 0) return 0;
     if (grid <= 0) grid = 256;
@@ -806,28 +848,58 @@ int ccx_hip_gax(const double *wbar, const double *bx, double *gax, const int *tu
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int ccx_hip_gbx(const double *wbar, const double *axr, const double *axc, double *gbx, const int *tuples, int t0,
-                const int *plist, const int *poff, const int *ent, int npair, int no, int nup, int kp, void *stream)
+void ccx_hip_gbx_geometry(int nup, int kp, int *nchunk, int *ngg, int *gtr)
 {
     const int nb = nup / TB;
-    const int ngg = (nb + 13) / 14;
-    const int gt = (nb + ngg - 1) / ngg;
-    static bool attr_set = false;
-    const size_t lds_max = sizeof(double) * (2 * 16 * (14 * 16 + 1) + 2 * GCH * BS_STR);
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)gbx_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_max) != hipSuccess)
-            return 2;
-        attr_set = true;
-    }
-    if (npair == 0) return 0;
+    *ngg = (nb + 13) / 14;                // gamma groups of at most 14 tiles
+    *gtr = 16 * ((nb + *ngg - 1) / *ngg); // rows of a partial tile
+    *nchunk = (kp + GCH - 1) / GCH;
+}
+
+size_t ccx_hip_gbx_part_doubles(int nup, int kp, int nent, int nsplit)
+{
+    int nchunk, ngg, gtr;
+    ccx_hip_gbx_geometry(nup, kp, &nchunk, &ngg, &gtr);
+    return (size_t)nent * nsplit * nchunk * ngg * gtr * GCH;
+}
+
+int ccx_hip_gbx(const double *wbar, const double *axr, const double *axc, double *gbx, double *part,
+                const int *tuples, int t0, const int *plist, const int *poff, const int *ent, int npair, int nent,
+                int nsplit, int no, int nup, int kp, void *stream)
+{
+    const int nb = nup / TB;
+    int nchunk, ngg, gtr;
+    ccx_hip_gbx_geometry(nup, kp, &nchunk, &ngg, &gtr);
+    const int gt = gtr / 16;
+    if (npair == 0 || nent == 0) return 0;
     GemmArgs a = {};
-    a.wbar = wbar; a.axr = axr; a.axc = axc; a.gbx = gbx; a.tuples = tuples; a.t0 = t0;
+    a.wbar = wbar; a.axr = axr; a.axc = axc; a.gbx = gbx; a.part = part; a.tuples = tuples; a.t0 = t0;
     a.ulist = plist; a.uoff = poff; a.ent = ent;
-    a.no = no; a.nup = nup; a.nb = nb; a.kp = kp; a.nchunk = (kp + GCH - 1) / GCH; a.gt = gt; a.ngg = ngg;
-    const size_t lds_bytes = sizeof(double) * (2 * 16 * (gt * 16 + 1) + 2 * GCH * BS_STR);
-    const long long grid = (long long)npair * a.nchunk * ngg;
-    gbx_kernel<<<(unsigned)grid, NTHR, lds_bytes, (hipStream_t)stream>>>(a);
+    a.no = no; a.nup = nup; a.nb = nb; a.kp = kp; a.nchunk = nchunk; a.gt = gt; a.ngg = ngg; a.nsplit = nsplit;
+    const long long grid = (long long)nent * nsplit * nchunk * ngg;
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    switch (gt) {
+    case 1: rc = launch_gbx<1>(a, grid, s); break;
+    case 2: rc = launch_gbx<2>(a, grid, s); break;
+    case 3: rc = launch_gbx<3>(a, grid, s); break;
+    case 4: rc = launch_gbx<4>(a, grid, s); break;
+    case 5: rc = launch_gbx<5>(a, grid, s); break;
+    case 6: rc = launch_gbx<6>(a, grid, s); break;
+    case 7: rc = launch_gbx<7>(a, grid, s); break;
+    case 8: rc = launch_gbx<8>(a, grid, s); break;
+    case 9: rc = launch_gbx<9>(a, grid, s); break;
+    case 10: rc = launch_gbx<10>(a, grid, s); break;
+    case 11: rc = launch_gbx<11>(a, grid, s); break;
+    case 12: rc = launch_gbx<12>(a, grid, s); break;
+    case 13: rc = launch_gbx<13>(a, grid, s); break;
+    default: rc = launch_gbx<14>(a, grid, s); break;
+    }
+    if (rc != 0) return rc;
+    const long long total = (long long)npair * nup * kp;
+    long long g2 = (total + 255) / 256;
+    if (g2 > 65536) g2 = 65536;
+    gbx_reduce_kernel<<<(unsigned)g2, 256, 0, s>>>(part, gbx, plist, poff, npair, nup, kp, nchunk, ngg, gtr, nsplit);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -33,12 +33,15 @@ int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tup
 int ccx_hip_adj_lds_bytes(void);
 int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const double *voep, const double *eh,
                       const double *ep, const double *t1, const int *tuples, const int *blocks, double *e_part,
-                      double *v1_part, double *wbar, double *slots, unsigned int *counter, int no, int nu, int nup,
-                      int kp, int ntup, int nblk, int t0, int tcount, int grid, void *stream);
+                      double *v1_part, double *wbar, double *slots, double *wstage, unsigned int *counter, int no,
+                      int nu, int nup, int kp, int ntup, int nblk, int t0, int tcount, int grid, void *stream);
+size_t ccx_hip_adj_stage_doubles(void);
 int ccx_hip_gax(const double *wbar, const double *bx, double *gax, const int *tuples, int t0, const int *xlist,
                 const int *xoff, const int *ent, int nx, int no, int nup, int kp, void *stream);
-int ccx_hip_gbx(const double *wbar, const double *axr, const double *axc, double *gbx, const int *tuples, int t0,
-                const int *plist, const int *poff, const int *ent, int npair, int no, int nup, int kp, void *stream);
+size_t ccx_hip_gbx_part_doubles(int nup, int kp, int nent, int nsplit);
+int ccx_hip_gbx(const double *wbar, const double *axr, const double *axc, double *gbx, double *part,
+                const int *tuples, int t0, const int *plist, const int *poff, const int *ent, int npair, int nent,
+                int nsplit, int no, int nup, int kp, void *stream);
 int ccx_hip_dvoe_reduce(const double *slots, double *dvoep, const int *plist, const int *poff, const int *ent,
                         int np, int no, int nup, void *stream);
 int ccx_hip_unpack_grad(const double *gax, const double *gbx, const double *dvoep, const double *v1, double *dvvvo,

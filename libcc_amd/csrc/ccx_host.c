@@ -410,6 +410,15 @@ static const int PERM_H[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2,
 
 /* CSR lists of (tl, code) entries grouped by key, keys in increasing order, entries in
  * (tl, code) order: list[u], off[u..u+1] for the nonzero keys only.  Returns #keys. */
+/* panel splits of the gbx back-contraction: enough work groups to fill the GPU when a
+ * batch holds few entries */
+static int gbx_nsplit(int64_t nent, int64_t nb)
+{
+    int64_t ns = nent > 0 ? (2048 + nent - 1) / nent : 1;
+    if (ns > nb) ns = nb;
+    return ns < 1 ? 1 : (int)ns;
+}
+
 static int build_csr(int nkeys, int n, const int *key, const int *tl, const int *code, int *list, int *off,
                      int *ent, int *count)
 {
@@ -441,7 +450,8 @@ int ccx_gradient(ccx_ctx *c, double *ets, double *etd, double *dt1, double *dt2,
     const int64_t no = c->no, nu = c->nu, nup = c->nup, nb = c->nb, kp = c->kp;
     const int64_t ntup = c->ntup, nblk = c->nblk;
     int rc = CCX_OK;
-    double *d_gax = NULL, *d_gbx = NULL, *d_dvoep = NULL, *d_wbar = NULL, *d_slots = NULL;
+    double *d_gax = NULL, *d_gbx = NULL, *d_dvoep = NULL, *d_wbar = NULL, *d_slots = NULL, *d_wstage = NULL;
+    double *d_gpart = NULL;
     double *d_out[5] = {NULL, NULL, NULL, NULL, NULL};
     int *d_lists = NULL, *h_lists = NULL, *h_tmp = NULL;
     double host_out[2] = {0.0, 0.0};
@@ -466,6 +476,12 @@ int ccx_gradient(ccx_ctx *c, double *ets, double *etd, double *dt1, double *dt2,
     DALLOC(d_wbar, tb * nup * nup * nup);
     DALLOC(d_slots, tb * 6 * nb * nb * nb * 256);
     DALLOC(d_lists, lsz);
+    DALLOC(d_wstage, (int64_t)grid_size(c, tb * nblk) * (int64_t)ccx_hip_adj_stage_doubles());
+    {   /* gbx partial tiles: the largest entries x splits over the full and the last batch */
+        const int64_t last = ntup > 0 ? ntup - (ntup - 1) / tb * tb : 1;
+        const int64_t u1 = 6 * tb * gbx_nsplit(6 * tb, nb), u2 = 6 * last * gbx_nsplit(6 * last, nb);
+        DALLOC(d_gpart, (int64_t)ccx_hip_gbx_part_doubles((int)nup, (int)kp, (int)(u1 > u2 ? u1 : u2), 1));
+    }
     if (ccx_hip_zero(d_gax, sizeof(double) * (size_t)(no * nup * nup * kp), c->stream) != 0 ||
         ccx_hip_zero(d_gbx, sizeof(double) * (size_t)(no * no * nup * kp), c->stream) != 0 ||
         ccx_hip_zero(d_dvoep, sizeof(double) * (size_t)(no * no * nup * nup), c->stream) != 0) {
@@ -503,13 +519,15 @@ int ccx_gradient(ccx_ctx *c, double *ets, double *etd, double *dt1, double *dt2,
             dl[kind][2] = dl[kind][1] + (nkey + 1);
         }
         if (ccx_hip_adj_items(c->d_axr, c->d_axc, c->d_bx, c->d_voep, c->d_eh, c->d_ep, c->d_t1, c->d_tuples,
-                              c->d_blocks, c->d_epart, c->d_v1part, d_wbar, d_slots, c->d_counter, (int)no, (int)nu,
+                              c->d_blocks, c->d_epart, c->d_v1part, d_wbar, d_slots, d_wstage, c->d_counter, (int)no,
+                              (int)nu,
                               (int)nup, (int)kp, (int)ntup, (int)nblk, (int)t0, tcount,
                               grid_size(c, (int64_t)tcount * nblk), c->stream) != 0 ||
             ccx_hip_gax(d_wbar, c->d_bx, d_gax, c->d_tuples, (int)t0, dl[0][0], dl[0][1], dl[0][2], nkeys[0], (int)no,
                         (int)nup, (int)kp, c->stream) != 0 ||
-            ccx_hip_gbx(d_wbar, c->d_axr, c->d_axc, d_gbx, c->d_tuples, (int)t0, dl[1][0], dl[1][1], dl[1][2],
-                        nkeys[1], (int)no, (int)nup, (int)kp, c->stream) != 0 ||
+            ccx_hip_gbx(d_wbar, c->d_axr, c->d_axc, d_gbx, d_gpart, c->d_tuples, (int)t0, dl[1][0], dl[1][1],
+                        dl[1][2], nkeys[1], 6 * tcount, gbx_nsplit(6 * tcount, nb), (int)no, (int)nup, (int)kp,
+                        c->stream) != 0 ||
             ccx_hip_dvoe_reduce(d_slots, d_dvoep, dl[2][0], dl[2][1], dl[2][2], nkeys[2], (int)no, (int)nup,
                                 c->stream) != 0) {
             rc = CCX_ERR_DEVICE;
@@ -554,6 +572,8 @@ fail:
     ccx_hip_sync(c->stream);
     ccx_hip_free(d_gax); ccx_hip_free(d_gbx); ccx_hip_free(d_dvoep); ccx_hip_free(d_wbar); ccx_hip_free(d_slots);
     ccx_hip_free(d_lists);
+    ccx_hip_free(d_wstage);
+    ccx_hip_free(d_gpart);
     for (int q = 0; q < 5; ++q) ccx_hip_free(d_out[q]);
     free(h_tmp);
     free(h_lists);

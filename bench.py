@@ -34,6 +34,43 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+PROFILE = os.path.join(ROOT, "profiles", "triples_kernel_fp64_no24_nu216.md")
+
+
+def measured_traffic(config: str):
+    """HBM bytes per launch of the persistent FP64 kernel at no=24, nu=216, as the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary records it (copy-calibrated), else None."""
+    if config != "c4" or not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def rocpd_summary(paths: list) -> None:
+    """Print, as markdown, per-kernel dispatch statistics and counter sums of rocprofv3
+    result databases (rocpd SQLite): the figures the committed profile summaries quote."""
+    import sqlite3
+
+    for path in paths:
+        c = sqlite3.connect(path)
+        print(f"### {os.path.basename(path)}\n")
+        print("| kernel | dispatches | total ms | average ms | % |\n|---|---|---|---|---|")
+        for name, calls, total, avg, pct in c.execute("select * from top_kernels"):
+            print(f"| `{name[:90]}` | {calls} | {total / 1e6:.3f} | {avg / 1e6:.3f} | {pct:.1f} |")
+        try:
+            rows = c.execute("select kernel_name, counter_name, sum(value), count(*), sum(duration) "
+                             "from counters_collection group by kernel_name, counter_name").fetchall()
+        except sqlite3.OperationalError:
+            rows = []
+        if rows:
+            print("\n| kernel | counter | sum over dispatches | dispatches | sum of durations ms |\n|---|---|---|---|---|")
+            for k, cn, val, n, dur in rows:
+                print(f"| `{k[:90]}` | {cn} | {val:.6g} | {n} | {dur / 1e6:.3f} |")
+        print()
+
+
 def cpu_baseline(inp_dev: dict, no: int, nu: int, budget_s: float, threads: int) -> dict | None:
     """Reference per-tuple routines on a bounded sample of distinct tuples, extrapolated
     to the whole tuple set by FLOP weight (a degenerate tuple is half a distinct one)."""
@@ -78,7 +115,12 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--rocpd-summary", nargs="+", default=None, metavar="DB",
+                    help="only summarize rocprofv3 result databases as markdown")
     args = ap.parse_args()
+    if args.rocpd_summary:
+        rocpd_summary(args.rocpd_summary)
+        return
     if args.copy_probe_mib:
         import torch
 
@@ -149,7 +191,7 @@ def main() -> None:
         "tuples_per_s": n_tup / (ms_step / 1e3),
         "kernel_ms": kernel_s * 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.config),
                      "flops_per_launch": flops, "executed_flops_per_launch": exec_flops, "work_items": items},
         "cpu_baseline": None,
     }

@@ -387,6 +387,8 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
 // back-contraction GEMMs (FP64 MFMA, LDS-staged, double buffered)
 // ---------------------------------------------------------------------------------
 namespace {
+constexpr int GTHR = 512;                 // GEMM work group: 8 waves, 2 per SIMD
+constexpr int GBPW = 2;                   // gax: rows b per wave
 constexpr int GCH = 128;                  // K columns per work group (8 MFMA tiles)
 constexpr int GCT = GCH / 16;
 constexpr int BS_STR = 17;                // padded row of the staged BX / AX slices
@@ -409,7 +411,7 @@ struct GemmArgs {
 // gAX[x](R0 16 x R1 16 rows, K chunk) += sum over entries (t,sigma) with t[xp] = x and over
 // the gamma blocks G of Wbar_t(e_u = a, e_v = b, e_wf = g) BX[t[yp],t[zp]](g, K).
 // Wave w owns rows b in {2w, 2w+1}, all 8 K tiles of the chunk.
-__global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
+__global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
 {
     extern __shared__ double lds[];
     double *WS = lds;                     // [2][WB_BLK]
@@ -427,9 +429,9 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
     const long long cube = (long long)a.nb * a.nb * a.nb * 4096;
     const int nsteps = (e_end - e_beg) * a.nb;
 
-    d4 acc[BPW][GCT];
+    d4 acc[GBPW][GCT];
 #pragma unroll
-    for (int bb = 0; bb < BPW; ++bb)
+    for (int bb = 0; bb < GBPW; ++bb)
 #pragma unroll
         for (int ct = 0; ct < GCT; ++ct) acc[bb][ct] = d4{0, 0, 0, 0};
 
@@ -446,13 +448,13 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
         const double *wsrc = a.wbar + tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const double2 v = reinterpret_cast<const double2 *>(wsrc)[tid + k * NTHR];
+            const double2 v = reinterpret_cast<const double2 *>(wsrc)[tid + k * GTHR];
             rw[2 * k] = v.x; rw[2 * k + 1] = v.y;
         }
         const double *bsrc = a.bx + (((long long)tpos[S.yp] * a.no + tpos[S.zp]) * a.nb + G) * (16LL * a.kp);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const int idx = 2 * (tid + k * NTHR);           // element of the 128 x 16 slice
+            const int idx = 2 * (tid + k * GTHR);           // element of the 128 x 16 slice
             const int Kl = idx >> 4;
             double2 v = {0.0, 0.0};
             if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(bsrc + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
@@ -463,7 +465,7 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
         double *W = WS + buf * WB_BLK;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int idx = 2 * (tid + k * NTHR);           // (a,b,c) with c even
+            const int idx = 2 * (tid + k * GTHR);           // (a,b,c) with c even
             const int aa = idx >> 8, b = (idx >> 4) & 15, c = idx & 15;
             W[aa * WB_STR_A + b * WB_STR_B + c] = rw[2 * k];
             W[aa * WB_STR_A + b * WB_STR_B + c + 1] = rw[2 * k + 1];
@@ -471,7 +473,7 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
         double *Bs = BS + buf * GCH * BS_STR;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const int idx = 2 * (tid + k * NTHR);
+            const int idx = 2 * (tid + k * GTHR);
             const int Kl = idx >> 4, gl = idx & 15;
             Bs[Kl * BS_STR + gl] = rb[2 * k];
             Bs[Kl * BS_STR + gl + 1] = rb[2 * k + 1];
@@ -495,14 +497,14 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
 #pragma unroll
         for (int kq = 0; kq < 4; ++kq) {
             const int g = 4 * kq + (lane >> 4);
-            double af[BPW];
+            double af[GBPW];
 #pragma unroll
-            for (int bb = 0; bb < BPW; ++bb) af[bb] = W[(lane & 15) * su + (wave * BPW + bb) * sv + g * sw];
+            for (int bb = 0; bb < GBPW; ++bb) af[bb] = W[(lane & 15) * su + (wave * GBPW + bb) * sv + g * sw];
 #pragma unroll
             for (int ct = 0; ct < GCT; ++ct) {
                 const double bf = Bs[(ct * 16 + (lane & 15)) * BS_STR + g];
 #pragma unroll
-                for (int bb = 0; bb < BPW; ++bb) acc[bb][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bb], bf, acc[bb][ct], 0, 0, 0);
+                for (int bb = 0; bb < GBPW; ++bb) acc[bb][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bb], bf, acc[bb][ct], 0, 0, 0);
             }
         }
         if (step + 1 < nsteps) stash(buf ^ 1);
@@ -510,7 +512,7 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
     }
     // read-add-write of the output tile: rows a = row(lane,v), b = 2*wave+bb; column K
 #pragma unroll
-    for (int bb = 0; bb < BPW; ++bb)
+    for (int bb = 0; bb < GBPW; ++bb)
 #pragma unroll
         for (int ct = 0; ct < GCT; ++ct) {
             const int K = K0 + ct * 16 + (lane & 15);
@@ -518,7 +520,7 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int al = Mfma<double>::row(lane, v);
-                double *o = a.gax + (((long long)x * a.nup + R0 * TB + al) * a.nup + R1 * TB + wave * BPW + bb) * a.kp + K;
+                double *o = a.gax + (((long long)x * a.nup + R0 * TB + al) * a.nup + R1 * TB + wave * GBPW + bb) * a.kp + K;
                 *o += acc[bb][ct][v];
             }
         }
@@ -533,7 +535,7 @@ __global__ void __launch_bounds__(NTHR, 1) gax_kernel(GemmArgs a)
 // else axis 1), each slice staged in LDS: Wbar (16 x gamma) and AX (16 x 128 K).  Wave w
 // owns K tile w and all gamma tiles.
 template <int GT>
-__global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
+__global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
 {
     extern __shared__ double lds[];
     constexpr int gstr = GT * 16 + 1;     // padded row of the staged Wbar slice
@@ -584,7 +586,7 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
         const int hstr = sa == 0 ? 16 : 256;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int idx = tid + k * NTHR;               // double2 index in the slice
+            const int idx = tid + k * GTHR;               // double2 index in the slice
             double2 v = {0.0, 0.0};
             if (idx < nw / 2) {
                 const int gt_l = idx >> 7, m = (idx & 127) * 2;
@@ -598,7 +600,7 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
                             ((((long long)tpos[S.xp] * a.nb + R0) * a.nb + R1) * 16 + sidx) * (16LL * a.kp);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const int idx = 2 * (tid + k * NTHR);
+            const int idx = 2 * (tid + k * GTHR);
             const int Kl = idx >> 4;
             double2 v = {0.0, 0.0};
             if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(src + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
@@ -609,7 +611,7 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
         double *W = WS + buf * 16 * gstr;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int idx = tid + k * NTHR;
+            const int idx = tid + k * GTHR;
             if (idx < nw / 2) {
                 const int gt_l = idx >> 7, m = (idx & 127) * 2;
                 const int h = m >> 4, l = m & 15;     // l, l+1 along axis 2
@@ -625,7 +627,7 @@ __global__ void __launch_bounds__(NTHR, 1) gbx_kernel(GemmArgs a)
         double *As = AS + buf * GCH * BS_STR;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const int idx = 2 * (tid + k * NTHR);
+            const int idx = 2 * (tid + k * GTHR);
             const int Kl = idx >> 4, o = idx & 15;
             As[Kl * BS_STR + o] = ra[2 * k];
             As[Kl * BS_STR + o + 1] = ra[2 * k + 1];
@@ -787,7 +789,7 @@ static int launch_gbx(const GemmArgs &a, long long grid, hipStream_t s)
             return 2;
         attr_set = true;
     }
-    gbx_kernel<GT><<<(unsigned)grid, NTHR, lds_bytes, s>>>(a);
+    gbx_kernel<GT><<<(unsigned)grid, GTHR, lds_bytes, s>>>(a);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -816,6 +818,7 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     g.blocks = blocks; g.e_part = e_part; g.v1_part = v1_part; g.wout = nullptr; g.counter = counter;
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
+    g.block_major = 0;
     ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.wstage = wstage; ga.t0 = t0; ga.tcount = tcount;
     const long long nlocal = (long long)tcount * nblk;
     if (nlocal == 0) return 0;
@@ -844,7 +847,7 @@ int ccx_hip_gax(const double *wbar, const double *bx, double *gax, const int *tu
     a.ulist = xlist; a.uoff = xoff; a.ent = ent;
     a.no = no; a.nup = nup; a.nb = nup / TB; a.kp = kp; a.nchunk = (kp + GCH - 1) / GCH;
     const long long grid = (long long)nx * a.nb * a.nb * a.nchunk;
-    gax_kernel<<<(unsigned)grid, NTHR, lds_bytes, (hipStream_t)stream>>>(a);
+    gax_kernel<<<(unsigned)grid, GTHR, lds_bytes, (hipStream_t)stream>>>(a);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

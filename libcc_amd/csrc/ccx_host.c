@@ -182,7 +182,7 @@ static int build_from_device(ccx_ctx *c, const double *d_eh, const double *d_ep_
     if (c->d_axr) goto allocated; /* buffers depend only on the dimensions: reuse */
     DALLOC(c->d_axr, no * nb * nb * kp * 256);
     DALLOC(c->d_axc, no * nb * nb * kp * 256);
-    DALLOC(c->d_counter, 4);
+    DALLOC(c->d_counter, 8);
     DALLOC(c->d_bx, no * no * kp * nup);
     DALLOC(c->d_voep, no * no * nup * nup);
     DALLOC(c->d_eh, no);

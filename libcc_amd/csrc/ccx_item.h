@@ -138,6 +138,7 @@ struct TriplesArgs {
     double *wout;          // W_ijk cube (nu^3, column-major) of tuple 0, or null
     unsigned int *counter; // work queue head (zeroed before the launch)
     int no, nu, nup, nb, kp, ntup, nblk;
+    int block_major;       // work order: 1 block triple major, 0 tuple major
     long long nitems;
 };
 
@@ -244,14 +245,22 @@ __device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &i
     const int nkb = g.kp / KS;
     Frags<T> f0, f1;
     load_frags<PHASE, MODE, T>(pp, 0, lane, f0);
-    int kb = 0;
-    for (; kb + 2 <= nkb; kb += 2) {
+    // Every load is unconditional (the last prefetch re-reads step nkb-1), so the number of
+    // loads in flight is the same on every path and each step waits only for its own
+    // fragments: a conditional prefetch makes the wait counter drain completely.
+    // The scheduling barriers keep each prefetch group ahead of the MFMAs of the current
+    // step (the scheduler would otherwise sink loads next to their uses).
+    for (int kb = 0; kb + 2 <= nkb; kb += 2) {
         load_frags<PHASE, MODE, T>(pp, kb + 1, lane, f1);
+        __builtin_amdgcn_sched_barrier(0);
         mfma_step<PHASE, MODE, T>(f0, acc);
-        if (kb + 2 < nkb) load_frags<PHASE, MODE, T>(pp, kb + 2, lane, f0);
+        __builtin_amdgcn_sched_barrier(0);
+        load_frags<PHASE, MODE, T>(pp, kb + 2 < nkb ? kb + 2 : nkb - 1, lane, f0);
+        __builtin_amdgcn_sched_barrier(0);
         mfma_step<PHASE, MODE, T>(f1, acc);
+        __builtin_amdgcn_sched_barrier(0);
     }
-    if (kb < nkb) mfma_step<PHASE, MODE, T>(f0, acc);
+    if (nkb & 1) mfma_step<PHASE, MODE, T>(f0, acc);
 }
 
 __device__ __forceinline__ double wave_sum(double v)

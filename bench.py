@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "(T) time-to-solution & matrix-peak %; split-prec vs FP64 at |ΔE(T)|<1 µEh"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 CONFIGS = {"c1": (4, 36, 1001), "c2": (4, 87, 1002), "c3": (8, 120, 1003), "c4": (24, 216, 1004),
            "c5": (30, 400, 1005)}
 
@@ -56,9 +57,9 @@ def rocpd_summary(paths: list) -> None:
     for path in paths:
         c = sqlite3.connect(path)
         print(f"### {os.path.basename(path)}\n")
-        print("| kernel | dispatches | total ms | average ms | % |\n|---|---|---|---|---|")
-        for name, calls, total, avg, pct in c.execute("select * from top_kernels"):
-            print(f"| `{name[:90]}` | {calls} | {total / 1e6:.3f} | {avg / 1e6:.3f} | {pct:.1f} |")
+        print("| kernel | dispatches | total s | average ms | % |\n|---|---|---|---|---|")
+        for name, calls, total, avg, pct in c.execute("select * from top_kernels"):   # microseconds
+            print(f"| `{name[:90]}` | {calls} | {total / 1e6:.3f} | {avg / 1e3:.3f} | {pct:.1f} |")
         try:
             rows = c.execute("select kernel_name, counter_name, sum(value), count(*), sum(duration) "
                              "from counters_collection group by kernel_name, counter_name").fetchall()
@@ -117,6 +118,10 @@ def main() -> None:
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--rocpd-summary", nargs="+", default=None, metavar="DB",
                     help="only summarize rocprofv3 result databases as markdown")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run one adjoint (gradient) evaluation and print its timing")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other arithmetic forms and the adjoint in the JSON line")
     args = ap.parse_args()
     if args.rocpd_summary:
         rocpd_summary(args.rocpd_summary)
@@ -147,6 +152,12 @@ def main() -> None:
     inp = synth.make_inputs_torch(no, nu, seed, "cuda:0")
     torch.cuda.synchronize()
     ctx = TriplesContext(no, nu)
+    if args.adjoint_only:
+        ctx.load(inp)
+        g = ctx.gradient(device=True)
+        print(json.dumps({"adjoint_ms": ctx.timing()[0], "e_t": g["e"], "no": no, "nu": nu}), flush=True)
+        ctx.close()
+        return
     log("warmup")
     ref = None
     for _ in range(args.warmup):
@@ -195,6 +206,26 @@ def main() -> None:
                      "flops_per_launch": flops, "executed_flops_per_launch": exec_flops, "work_items": items},
         "cpu_baseline": None,
     }
+    if not args.no_extra:
+        # other arithmetic forms of the same solve: kernel time and |dE| against FP64
+        out["forms"] = {"fp64": {"kernel_ms": kernel_s * 1e3, "e_t": r["e"], "abs_de": 0.0,
+                                 "frac_of_peak": achieved / PEAK_FP64_TFLOPS,
+                                 "peak_tflops": PEAK_FP64_TFLOPS}}
+        for form, peak in (("fp32", PEAK_FP32_TFLOPS),):
+            ctx.energy(form)
+            rf = ctx.energy(form)
+            kf = ctx.timing()[0] / 1e3
+            out["forms"][form] = {"kernel_ms": kf * 1e3, "e_t": rf["e"], "abs_de": abs(rf["e"] - r["e"]),
+                                  "speedup_vs_fp64": kernel_s / kf,
+                                  "frac_of_peak": flops / kf / 1e12 / peak, "peak_tflops": peak}
+        # the adjoint: dE/d(T1, T2, VM, VOE, VVVO) with per-item recomputation of W
+        log("adjoint")
+        g = ctx.gradient(device=True)
+        adj_ms = ctx.timing()[0]
+        del g
+        out["adjoint"] = {"ms": adj_ms, "ratio_to_forward": adj_ms / (kernel_s * 1e3),
+                          "flops": 3.0 * flops,
+                          "frac_of_fp64_peak": 3.0 * flops / (adj_ms / 1e3) / 1e12 / PEAK_FP64_TFLOPS}
     ctx.close()
     if not args.no_cpu:
         log("cpu baseline")

@@ -136,6 +136,32 @@ void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const do
                 const double *vm_ki, const double *vm_jk, const double *vm_kj, const double *ve_i,
                 const double *ve_j, const double *ve_k, double *v3);
 
+/* t1wt3_ijk(i,j,k,no,nu,v3,voe_ij,voe_ji,voe_ik,voe_ki,voe_jk,voe_kj,t1,eh,ep):
+ * src/triples-ijk/t1wt3.f90:18-109.  From the tuple's W cube in v3 (nu^3), adds the tuple's E[T]
+ * to the module etd (halved when i=j or j=k, t1wt3.f90:98-102) and its v1 rows i, j, k to t1,
+ * which is the v1 accumulator (nu,no) as the driver passes it.  Runs on the GPU. */
+void t1wt3_ijk_(int64_t *i, int64_t *j, int64_t *k, int64_t *no, int64_t *nu, const double *v3,
+                const double *voe_ij, const double *voe_ji, const double *voe_ik, const double *voe_ki,
+                const double *voe_jk, const double *voe_kj, double *t1, const double *eh, const double *ep);
+
+/* ddcc_t_ijk_acc(no,nu,i,j,k,t1,t2,vm,v3,voe,eh,ep,ve_i,ve_j,ve_k): src/triples-ijk/ddcc_t_ijk_big.f:1-21.
+ * One distinct tuple of the driver loop: W into v3, E[T] into the module etd, v1 rows into t1
+ * (the v1 accumulator, nu x no).  Full t2, vm, voe; ve_x = VE_x(d,p,q) = VVVO(d,q,p,x). */
+void ddcc_t_ijk_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, int64_t *k, double *t1, double *t2,
+                     double *vm, double *v3, double *voe, double *eh, double *ep, double *ve_i, double *ve_j,
+                     double *ve_k);
+
+/* ddcc_t_ijj_acc / ddcc_t_iij_acc(no,nu,i,j,t1,t2,vm,v3,t3,voe,o1,eh,ep,temp,ve_i,ve_j):
+ * src/triples-ijk/ddcc_t_ijj_iij_big.f:18-94.  The degenerate tuples (i,j,j) and (i,i,j):
+ * E[T] with weight 1/2 into etd, v1 rows into t1 (the accumulator), W into v3.  t3, o1
+ * (the T1 amplitudes) and temp are the reference's scratch and are not read. */
+void ddcc_t_ijj_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t1, double *t2, double *vm,
+                     double *v3, double *t3, double *voe, double *o1, double *eh, double *ep, double *temp,
+                     double *ve_i, double *ve_j);
+void ddcc_t_iij_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t1, double *t2, double *vm,
+                     double *v3, double *t3, double *voe, double *o1, double *eh, double *ep, double *temp,
+                     double *ve_i, double *ve_j);
+
 /* trant3 family: in-place permutations of an n^3 cube, new(x,y,z) = old(s(x,y,z)).
  *   trant3_1_ (x,z,y)  trant3_3_ (z,y,x)  trant3_4_ (y,z,x)  trant3_5_ (z,x,y)
  * src/common/common_cc.f90:62-156.  trant3_acc_(v,nu,id) and trant3_smp_(v,nu,id) select by

@@ -79,6 +79,13 @@ def lib() -> ctypes.CDLL:
     L.ccx_get_etd_.restype = ctypes.c_double
     L.ijk_tuple_.argtypes = [pi64, pi64, pi64] + [dp] * 13
     L.ijk_tuple_.restype = None
+    L.t1wt3_ijk_.argtypes = [pi64] * 5 + [dp] * 10
+    L.t1wt3_ijk_.restype = None
+    L.ddcc_t_ijk_acc_.argtypes = [pi64] * 5 + [dp] * 10
+    L.ddcc_t_ijk_acc_.restype = None
+    for name in ("ddcc_t_ijj_acc_", "ddcc_t_iij_acc_"):
+        getattr(L, name).argtypes = [pi64] * 4 + [dp] * 12
+        getattr(L, name).restype = None
     for name in ("trant3_1_", "trant3_3_", "trant3_4_", "trant3_5_"):
         getattr(L, name).argtypes = [pi64, dp]
         getattr(L, name).restype = None

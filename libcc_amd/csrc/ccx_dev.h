@@ -29,6 +29,9 @@ int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tup
                      const double *t1, double *v1t, double *v1, double *out, int no, int nu, int ntup, int nblk,
                      void *stream);
 
+int ccx_hip_t1wt3(const double *w, const double *voe6, double dijk, const double *ep, int nu, double *x3,
+                  double *v1p, void *stream);
+
 /* adjoint (ccx_adjoint.hip) */
 int ccx_hip_adj_lds_bytes(void);
 int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const double *voep, const double *eh,

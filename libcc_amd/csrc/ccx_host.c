@@ -687,18 +687,17 @@ void cc_triples_(double *eh, double *ep, double *v1, double *t1, double *t2, dou
     legacy_triples(g_d_vvvo, eh, ep, v1, t1, t2, vm, voe, NULL, NULL);
 }
 
-void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const double *t2_j, const double *t2_k,
-                const double *vm_ij, const double *vm_ji, const double *vm_ik, const double *vm_ki,
-                const double *vm_jk, const double *vm_kj, const double *ve_i, const double *ve_j,
-                const double *ve_k, double *v3)
+/* W_ijk (0-based tuple) into v3 (nu^3) from the slices the reference's ijk_tuple receives,
+ * on the GPU: full-size arrays hold only those slices (the rest is zero), then the
+ * single-tuple W of the persistent kernel; W(a,a,a) = 0 as zerot3 leaves it. */
+static int tuple_w_host(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk, const double *t2_i,
+                        const double *t2_j, const double *t2_k, const double *vm_ij, const double *vm_ji,
+                        const double *vm_ik, const double *vm_ki, const double *vm_jk, const double *vm_kj,
+                        const double *ve_i, const double *ve_j, const double *ve_k, double *v3)
 {
-    if (!i || !j || !k || !t2_i || !t2_j || !t2_k || !vm_ij || !vm_ji || !vm_ik || !vm_ki || !vm_jk || !vm_kj ||
-        !ve_i || !ve_j || !ve_k || !v3) { g_status = CCX_ERR_ARG; return; }
-    const int64_t no = g_no, nu = g_nu, nu2 = nu * nu;
-    if (no < 1 || nu < 1) { g_status = CCX_ERR_STATE; return; }
-    const int64_t ii = *i - 1, jj = *j - 1, kk = *k - 1;
-    if (ii < 0 || jj < 0 || kk < 0 || ii >= no || jj >= no || kk >= no) { g_status = CCX_ERR_ARG; return; }
-    /* Assemble full-size arrays holding only the slices the tuple reads; the rest is zero. */
+    const int64_t nu2 = nu * nu;
+    if (no < 1 || nu < 1) return CCX_ERR_STATE;
+    if (ii < 0 || jj < 0 || kk < 0 || ii >= no || jj >= no || kk >= no) return CCX_ERR_ARG;
     const size_t n_t2 = (size_t)(nu2 * no * no), n_vm = (size_t)(no * nu * no * no);
     double *t2 = (double *)calloc(n_t2, sizeof(double)), *vm = (double *)calloc(n_vm, sizeof(double));
     double *voe = (double *)calloc(n_t2, sizeof(double));
@@ -727,13 +726,138 @@ void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const do
     }
     rc = ccx_create(no, nu, &c);
     if (rc == CCX_OK) rc = ccx_load_host(c, eh, ep, t1, t2, vm, voe, vvvo, 0);
-    if (rc == CCX_OK) rc = ccx_tuple_w(c, *i, *j, *k, v3);
+    if (rc == CCX_OK) rc = ccx_tuple_w(c, ii + 1, jj + 1, kk + 1, v3);
     if (rc == CCX_OK)
         for (int64_t a = 0; a < nu; ++a) v3[a + nu * (a + nu * a)] = 0.0; /* zerot3 */
 done:
     if (c) ccx_destroy(c);
     free(t2); free(vm); free(voe); free(vvvo); free(eh); free(ep); free(t1);
-    g_status = rc;
+    return rc;
+}
+
+void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const double *t2_j, const double *t2_k,
+                const double *vm_ij, const double *vm_ji, const double *vm_ik, const double *vm_ki,
+                const double *vm_jk, const double *vm_kj, const double *ve_i, const double *ve_j,
+                const double *ve_k, double *v3)
+{
+    if (!i || !j || !k || !t2_i || !t2_j || !t2_k || !vm_ij || !vm_ji || !vm_ik || !vm_ki || !vm_jk || !vm_kj ||
+        !ve_i || !ve_j || !ve_k || !v3) { g_status = CCX_ERR_ARG; return; }
+    g_status = tuple_w_host(g_no, g_nu, *i - 1, *j - 1, *k - 1, t2_i, t2_j, t2_k, vm_ij, vm_ji, vm_ik, vm_ki, vm_jk,
+                            vm_kj, ve_i, ve_j, ve_k, v3);
+}
+
+/* E[T] and v1 of one tuple (0-based i,j,k) from its W cube on the GPU (t1wt3_kernel):
+ * etd += efac * x3, v1(:,i/j/k) += vfac * (row sums); v1 is (nu,no). */
+static int t1wt3_host(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk, const double *v3,
+                      const double *voe6[6], double *v1, const double *eh, const double *ep, double efac,
+                      double vfac)
+{
+    if (no < 1 || nu < 1) return CCX_ERR_STATE;
+    if (ii < 0 || jj < 0 || kk < 0 || ii >= no || jj >= no || kk >= no) return CCX_ERR_ARG;
+    const size_t nu2 = (size_t)(nu * nu), nu3 = nu2 * (size_t)nu;
+    int rc = CCX_OK;
+    double *d_w = NULL, *d_voe = NULL, *d_ep = NULL, *d_x3 = NULL, *d_v1p = NULL;
+    double *x3 = (double *)malloc(sizeof(double) * (size_t)nu), *v1p = (double *)malloc(sizeof(double) * 3 * (size_t)nu);
+    if (!x3 || !v1p) { rc = CCX_ERR_ALLOC; goto fail; }
+    DALLOC(d_w, nu3);
+    DALLOC(d_voe, 6 * nu2);
+    DALLOC(d_ep, nu);
+    DALLOC(d_x3, nu);
+    DALLOC(d_v1p, 3 * nu);
+    if (ccx_hip_memcpy_h2d(d_w, v3, sizeof(double) * nu3, NULL) != 0 ||
+        ccx_hip_memcpy_h2d(d_ep, ep, sizeof(double) * (size_t)nu, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    for (int q = 0; q < 6; ++q)
+        if (ccx_hip_memcpy_h2d(d_voe + q * nu2, voe6[q], sizeof(double) * nu2, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    if (ccx_hip_t1wt3(d_w, d_voe, eh[ii] + eh[jj] + eh[kk], d_ep, (int)nu, d_x3, d_v1p, NULL) != 0 ||
+        ccx_hip_memcpy_d2h(x3, d_x3, sizeof(double) * (size_t)nu, NULL) != 0 ||
+        ccx_hip_memcpy_d2h(v1p, d_v1p, sizeof(double) * 3 * (size_t)nu, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    {
+        double e = 0.0;
+        for (int64_t a = 0; a < nu; ++a) e += x3[a];
+        g_etd += efac * e;
+        const int64_t rows[3] = {ii, jj, kk};
+        for (int r = 0; r < 3; ++r)
+            for (int64_t a = 0; a < nu; ++a) v1[a + nu * rows[r]] += vfac * v1p[3 * a + r];
+    }
+fail:
+    ccx_hip_free(d_w); ccx_hip_free(d_voe); ccx_hip_free(d_ep); ccx_hip_free(d_x3); ccx_hip_free(d_v1p);
+    free(x3); free(v1p);
+    return rc;
+}
+
+void t1wt3_ijk_(int64_t *i, int64_t *j, int64_t *k, int64_t *no, int64_t *nu, const double *v3,
+                const double *voe_ij, const double *voe_ji, const double *voe_ik, const double *voe_ki,
+                const double *voe_jk, const double *voe_kj, double *t1, const double *eh, const double *ep)
+{
+    if (!i || !j || !k || !no || !nu || !v3 || !voe_ij || !voe_ji || !voe_ik || !voe_ki || !voe_jk || !voe_kj ||
+        !t1 || !eh || !ep) { g_status = CCX_ERR_ARG; return; }
+    const double *voe6[6] = {voe_ij, voe_ji, voe_ik, voe_ki, voe_jk, voe_kj};
+    /* the reference halves the energy of a tuple with equal indices (t1wt3.f90:98-102) */
+    const double cf = (*i == *j || *j == *k) ? 0.5 : 1.0;
+    g_status = t1wt3_host(*no, *nu, *i - 1, *j - 1, *k - 1, v3, voe6, t1, eh, ep, cf, 1.0);
+}
+
+/* one tuple of the driver loop: W into v3, then etd and the v1 rows; efac/vfac are the
+ * tuple's weights (1 distinct; 1/2 for (i,j,j) and (i,i,j), whose six orderings are the
+ * three distinct ones twice) */
+static int tuple_step(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk, double *v1, const double *t2,
+                      const double *vm, double *v3, const double *voe, const double *eh, const double *ep,
+                      const double *ve_i, const double *ve_j, const double *ve_k, double fac)
+{
+    const int64_t nu2 = nu * nu, nou = no * nu;
+    const double *t2x[3], *vmx[6], *voex[6];
+    const int64_t xs[3] = {ii, jj, kk};
+    for (int p = 0; p < 3; ++p) t2x[p] = t2 + nu2 * no * xs[p];            /* T2(:,:,:,x) */
+    const int64_t px[6] = {ii, jj, ii, kk, jj, kk}, py[6] = {jj, ii, kk, ii, kk, jj};
+    for (int p = 0; p < 6; ++p) {
+        vmx[p] = vm + nou * (px[p] + no * py[p]);                          /* VM(:,:,x,y) */
+        voex[p] = voe + nu2 * (px[p] + no * py[p]);                        /* VOE(:,:,x,y) */
+    }
+    int rc = tuple_w_host(no, nu, ii, jj, kk, t2x[0], t2x[1], t2x[2], vmx[0], vmx[1], vmx[2], vmx[3], vmx[4], vmx[5],
+                          ve_i, ve_j, ve_k, v3);
+    if (rc == CCX_OK) rc = t1wt3_host(no, nu, ii, jj, kk, v3, voex, v1, eh, ep, fac, fac);
+    return rc;
+}
+
+/* ddcc_t_ijk_acc(no,nu,i,j,k,t1,t2,vm,v3,voe,eh,ep,ve_i,ve_j,ve_k): src/triples-ijk/ddcc_t_ijk_big.f:1-21.
+ * The argument named t1 is the v1 accumulator (nu,no), as the driver passes it. */
+void ddcc_t_ijk_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, int64_t *k, double *t1, double *t2,
+                     double *vm, double *v3, double *voe, double *eh, double *ep, double *ve_i, double *ve_j,
+                     double *ve_k)
+{
+    if (!no || !nu || !i || !j || !k || !t1 || !t2 || !vm || !v3 || !voe || !eh || !ep || !ve_i || !ve_j || !ve_k) {
+        g_status = CCX_ERR_ARG;
+        return;
+    }
+    g_status = tuple_step(*no, *nu, *i - 1, *j - 1, *k - 1, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_j, ve_k, 1.0);
+}
+
+/* ddcc_t_ijj_acc(no,nu,i,j,t1,t2,vm,v3,t3,voe,o1,eh,ep,temp,ve_i,ve_j) and the (i,i,j)
+ * twin: src/triples-ijk/ddcc_t_ijj_iij_big.f:18-94.  t1 is the v1 accumulator; t3, o1 and
+ * temp are the reference's scratch and T1 copy, not read here.  v3 receives W (the
+ * reference leaves it in a transformed scratch state). */
+void ddcc_t_ijj_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t1, double *t2, double *vm,
+                     double *v3, double *t3, double *voe, double *o1, double *eh, double *ep, double *temp,
+                     double *ve_i, double *ve_j)
+{
+    if (!no || !nu || !i || !j || !t1 || !t2 || !vm || !v3 || !voe || !eh || !ep || !ve_i || !ve_j) {
+        g_status = CCX_ERR_ARG;
+        return;
+    }
+    (void)t3; (void)o1; (void)temp;
+    g_status = tuple_step(*no, *nu, *i - 1, *j - 1, *j - 1, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_j, ve_j, 0.5);
+}
+
+void ddcc_t_iij_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t1, double *t2, double *vm,
+                     double *v3, double *t3, double *voe, double *o1, double *eh, double *ep, double *temp,
+                     double *ve_i, double *ve_j)
+{
+    if (!no || !nu || !i || !j || !t1 || !t2 || !vm || !v3 || !voe || !eh || !ep || !ve_i || !ve_j) {
+        g_status = CCX_ERR_ARG;
+        return;
+    }
+    (void)t3; (void)o1; (void)temp;
+    g_status = tuple_step(*no, *nu, *i - 1, *i - 1, *j - 1, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_i, ve_j, 0.5);
 }
 
 /* ------------------------------------------------------------------ permutations (host) */

@@ -391,6 +391,55 @@ __global__ void __launch_bounds__(1024) finalize_energy_kernel(const double *__r
     }
 }
 
+// E[T] and v1 of one tuple from a given W_ijk cube (nu^3, column-major), the computation of
+// t1wt3_ijk (src/triples-ijk/t1wt3.f90:53-102) as a full sum over (b,c) per work group a:
+//   x3[a]      = sum_{b,c, not a=b=c} W(abc) (8W(abc) - 4(W(acb)+W(cba)+W(bac)) + 2(W(bca)+W(cab))) / D
+//   v1p[a][0]  = sum_{b,c} (Z_ijk(abc) VOE_jk(bc) + Z_ikj(abc) VOE_kj(bc)) / D     (row i)
+//   v1p[a][1]  = sum_{b,c} (Z_jik(abc) VOE_ik(bc) + Z_kij(abc) VOE_ki(bc)) / D     (row j)
+//   v1p[a][2]  = sum_{b,c} (Z_jki(abc) VOE_ij(bc) + Z_kji(abc) VOE_ji(bc)) / D     (row k)
+// with Z_P the antisymmetric combination 2(W_P(abc)-W_P(bac)) - W_P(acb) + W_P(bca) of the
+// cube read in ordering P.  voe6 = VOE(:,:,x,y) for xy = ij, ji, ik, ki, jk, kj.  Fixed-order
+// tree reduction per work group: deterministic.
+__global__ void __launch_bounds__(256) t1wt3_kernel(const double *__restrict__ w, const double *__restrict__ voe6,
+                                                    double dijk, const double *__restrict__ ep, int nu,
+                                                    double *__restrict__ x3, double *__restrict__ v1p)
+{
+    __shared__ double sh[4][256];
+    const int a = blockIdx.x, tid = threadIdx.x;
+    const long long n2 = (long long)nu * nu;
+    auto W = [&](int x, int y, int z) { return w[x + (long long)nu * (y + (long long)nu * z)]; };
+    double e = 0.0, ri = 0.0, rj = 0.0, rk = 0.0;
+    for (long long bc = tid; bc < n2; bc += 256) {
+        const int b = (int)(bc % nu), c = (int)(bc / nu);
+        if (a == b && b == c) continue;
+        const double inv = 1.0 / (dijk - ep[a] - ep[b] - ep[c]);
+        const double abc = W(a, b, c), acb = W(a, c, b), bac = W(b, a, c);
+        const double bca = W(b, c, a), cab = W(c, a, b), cba = W(c, b, a);
+        e += abc * (8.0 * abc - 4.0 * (acb + cba + bac) + 2.0 * (bca + cab)) * inv;
+        const double z1 = 2.0 * (abc - bac) - acb + bca;   // ijk
+        const double z2 = 2.0 * (acb - bca) - abc + bac;   // ikj
+        const double z3 = 2.0 * (bac - abc) - cab + cba;   // jik
+        const double z4 = 2.0 * (bca - acb) - cba + cab;   // jki
+        const double z5 = 2.0 * (cab - cba) - bac + abc;   // kij
+        const double z6 = 2.0 * (cba - cab) - bca + acb;   // kji
+        const long long o = b + (long long)nu * c;
+        ri += (z1 * voe6[4 * n2 + o] + z2 * voe6[5 * n2 + o]) * inv;
+        rj += (z3 * voe6[2 * n2 + o] + z5 * voe6[3 * n2 + o]) * inv;
+        rk += (z4 * voe6[0 * n2 + o] + z6 * voe6[1 * n2 + o]) * inv;
+    }
+    sh[0][tid] = e; sh[1][tid] = ri; sh[2][tid] = rj; sh[3][tid] = rk;
+    __syncthreads();
+    for (int s2 = 128; s2 >= 1; s2 >>= 1) {
+        if (tid < s2)
+            for (int q = 0; q < 4; ++q) sh[q][tid] += sh[q][tid + s2];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        x3[a] = sh[0][0];
+        v1p[3 * a] = sh[1][0]; v1p[3 * a + 1] = sh[2][0]; v1p[3 * a + 2] = sh[3][0];
+    }
+}
+
 __global__ void zero_counter_kernel(unsigned int *c) { *c = 0u; }
 
 __global__ void to_f32_kernel(const double *__restrict__ src, float *__restrict__ dst, long long n)
@@ -521,6 +570,14 @@ int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tup
         if (hipMemsetAsync(v1, 0, sizeof(double) * nu * no, s) != hipSuccess) return 1;
     }
     finalize_energy_kernel<<<1, 1024, 0, s>>>(e_part, (long long)ntup * nblk, t1, v1, no, nu, out);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_t1wt3(const double *w, const double *voe6, double dijk, const double *ep, int nu, double *x3,
+                  double *v1p, void *stream)
+{
+    if (nu < 1) return 0;
+    t1wt3_kernel<<<nu, 256, 0, (hipStream_t)stream>>>(w, voe6, dijk, ep, nu, x3, v1p);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

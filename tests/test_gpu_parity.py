@@ -193,3 +193,81 @@ def test_nonfinite_input_rejected():
     with pytest.raises(Exception) as e:
         cc_triples(*[bad[k] for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")])
     assert "status 5" in str(e.value)
+
+
+def _ve(vvvo, x):
+    return np.asfortranarray(np.einsum("dqp->dpq", vvvo[:, :, :, x]))   # VE_x(d,p,q) = VVVO(d,q,p,x)
+
+
+@pytest.mark.parametrize("no,nu,seed", [(4, 6, 11), (5, 20, 31)])
+def test_legacy_per_tuple_entries_match_reference_routines(no, nu, seed):
+    """ddcc_t_ijk_acc_, ddcc_t_ijj_acc_, ddcc_t_iij_acc_ and t1wt3_ijk_ against the C restatement of
+    the same reference routines (oracle/port.c), one tuple at a time: v1 rows, etd, W."""
+    inp = synth.make_inputs(no, nu, seed)
+    L = lib()
+    f = np.asfortranarray
+    a = {k: f(inp[k].copy()) for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    scratch = [np.zeros(nu ** 3) for _ in range(3)]
+    # kind 0: (i,j,k); kind 1: (i,j,j); kind 2: (i,i,j) -- the degenerate kinds take (i, j)
+    for kind, (i, j, k) in [(0, (no - 1, 2, 1)), (1, (no - 1, 1, 0)), (2, (no - 1, 0, 0))]:
+        want = port.tuple_(inp, kind, i, j, k)
+        L.common_cc_init_(_i64(no), _i64(nu))
+        v1 = np.zeros((nu, no), order="F")
+        v3 = np.zeros((nu, nu, nu), order="F")
+        if kind == 0:
+            L.ddcc_t_ijk_acc_(_i64(no), _i64(nu), _i64(i + 1), _i64(j + 1), _i64(k + 1), _dp(v1), _dp(a["t2"]),
+                              _dp(a["vm"]), _dp(v3), _dp(a["voe"]), _dp(a["eh"]), _dp(a["ep"]),
+                              _dp(_ve(inp["vvvo"], i)), _dp(_ve(inp["vvvo"], j)), _dp(_ve(inp["vvvo"], k)))
+        else:
+            fn = L.ddcc_t_ijj_acc_ if kind == 1 else L.ddcc_t_iij_acc_
+            fn(_i64(no), _i64(nu), _i64(i + 1), _i64(j + 1), _dp(v1), _dp(a["t2"]), _dp(a["vm"]), _dp(v3),
+               _dp(scratch[0]), _dp(a["voe"]), _dp(a["t1"]), _dp(a["eh"]), _dp(a["ep"]), _dp(scratch[1]),
+               _dp(_ve(inp["vvvo"], i)), _dp(_ve(inp["vvvo"], j)))
+        assert L.ccx_last_status() == 0, kind
+        etd = L.ccx_get_etd_()
+        assert abs(etd - want["etd"]) <= 1e-12 * max(1.0, abs(want["etd"])), (kind, etd, want["etd"])
+        _v1_ok(v1, want["v1"])
+        if kind == 0:
+            np.testing.assert_allclose(v3, want["v3"], rtol=0, atol=1e-14 * np.abs(want["v3"]).max())
+            # the same tuple through t1wt3_ijk_ alone, from the reference's W
+            L.common_cc_init_(_i64(no), _i64(nu))
+            v1b = np.zeros((nu, no), order="F")
+            voe = inp["voe"]
+            sl = {(x, y): f(voe[:, :, x, y]) for x in (i, j, k) for y in (i, j, k) if x != y}
+            L.t1wt3_ijk_(_i64(i + 1), _i64(j + 1), _i64(k + 1), _i64(no), _i64(nu), _dp(f(want["v3"])),
+                         _dp(sl[(i, j)]), _dp(sl[(j, i)]), _dp(sl[(i, k)]), _dp(sl[(k, i)]), _dp(sl[(j, k)]),
+                         _dp(sl[(k, j)]), _dp(v1b), _dp(a["eh"]), _dp(a["ep"]))
+            assert L.ccx_last_status() == 0
+            assert abs(L.ccx_get_etd_() - want["etd"]) <= 1e-12 * max(1.0, abs(want["etd"]))
+            _v1_ok(v1b, want["v1"])
+
+
+def test_legacy_tuple_loop_equals_driver(golden):
+    """Driving the whole tuple loop through the per-tuple entries gives the driver's energy."""
+    no, nu, seed = 5, 7, 3
+    inp = synth.make_inputs(no, nu, seed)
+    L = lib()
+    f = np.asfortranarray
+    a = {k: f(inp[k].copy()) for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    ve = {x: _ve(inp["vvvo"], x) for x in range(no)}
+    v1 = np.zeros((nu, no), order="F")
+    v3 = np.zeros((nu, nu, nu), order="F")
+    s = [np.zeros(nu ** 3) for _ in range(2)]
+    L.common_cc_init_(_i64(no), _i64(nu))
+    for i in range(no):
+        for j in range(i):
+            for k in range(j):
+                L.ddcc_t_ijk_acc_(_i64(no), _i64(nu), _i64(i + 1), _i64(j + 1), _i64(k + 1), _dp(v1), _dp(a["t2"]),
+                                  _dp(a["vm"]), _dp(v3), _dp(a["voe"]), _dp(a["eh"]), _dp(a["ep"]), _dp(ve[i]),
+                                  _dp(ve[j]), _dp(ve[k]))
+                assert L.ccx_last_status() == 0
+    for i in range(no):
+        for j in range(i):
+            for fn, (x, y) in ((L.ddcc_t_ijj_acc_, (i, j)), (L.ddcc_t_iij_acc_, (i, j))):
+                fn(_i64(no), _i64(nu), _i64(x + 1), _i64(y + 1), _dp(v1), _dp(a["t2"]), _dp(a["vm"]), _dp(v3),
+                   _dp(s[0]), _dp(a["voe"]), _dp(a["t1"]), _dp(a["eh"]), _dp(a["ep"]), _dp(s[1]), _dp(ve[x]),
+                   _dp(ve[y]))
+                assert L.ccx_last_status() == 0
+    ets_want, etd_want = golden["e_5_7_3"]
+    assert abs(L.ccx_get_etd_() - etd_want) <= max(1e-9, 1e-12 * abs(etd_want))
+    _v1_ok(v1, golden["v1_5_7_3"])

@@ -120,6 +120,8 @@ def main() -> None:
                     help="only summarize rocprofv3 result databases as markdown")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run one adjoint (gradient) evaluation and print its timing")
+    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32"],
+                    help="arithmetic form of the timed steps (the headline metric is fp64)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other arithmetic forms and the adjoint in the JSON line")
     args = ap.parse_args()
@@ -162,14 +164,14 @@ def main() -> None:
     ref = None
     for _ in range(args.warmup):
         ctx.load(inp)
-        ref = ctx.energy("fp64")
+        ref = ctx.energy(args.form)
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
     kms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.load(inp)
-        r = ctx.energy("fp64")
+        r = ctx.energy(args.form)
         kms.append(ctx.timing()[0])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -181,6 +183,7 @@ def main() -> None:
     items, exec_flops = ctx.work_stats()
     n_tup = no * (no - 1) * (no - 2) // 6 + no * (no - 1)
     achieved = flops / kernel_s / 1e12
+    peak = PEAK_FP64_TFLOPS if args.form == "fp64" else PEAK_FP32_TFLOPS
     out = {
         "metric": METRIC,
         "value": ms_step / 1e3,
@@ -192,21 +195,22 @@ def main() -> None:
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.form == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": f"closed-shell (T) E(T)=E[T]+E_ST and v1, no={no} nu={nu}, "
                                f"{no * (no - 1) * (no - 2) // 6} distinct + {no * (no - 1)} degenerate tuples, "
-                               f"FP64 form, inputs resident in HBM", "no": no, "nu": nu, "seed": seed,
-                   "form": "fp64"},
+                               f"{args.form.upper()} form, inputs resident in HBM", "no": no, "nu": nu,
+                   "seed": seed, "form": args.form},
         "e_t": r["e"], "etd": r["etd"], "ets": r["ets"],
         "tuples_per_s": n_tup / (ms_step / 1e3),
         "kernel_ms": kernel_s * 1e3,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.config),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic(args.config) if args.form == "fp64" else None,
                      "flops_per_launch": flops, "executed_flops_per_launch": exec_flops, "work_items": items},
         "cpu_baseline": None,
     }
-    if not args.no_extra:
+    if not args.no_extra and args.form == "fp64":
         # other arithmetic forms of the same solve: kernel time and |dE| against FP64
         out["forms"] = {"fp64": {"kernel_ms": kernel_s * 1e3, "e_t": r["e"], "abs_de": 0.0,
                                  "frac_of_peak": achieved / PEAK_FP64_TFLOPS,

@@ -595,16 +595,17 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
             }
             rw[k] = v;
         }
-        // AX slice: row layout (fixed alpha = e_u) or column layout (fixed beta = e_v)
+        // AX slice: row layout (fixed alpha = e_u) or column layout (fixed beta = e_v); the
+        // slab interleaves the outer indices 2w, 2w+1 per lane: element (K, o) of outer index
+        // sidx sits at ((sidx/2 * kp/4 + K/4) * 64 + (K%4)*16 + o) * 2 + sidx%2 of the panel
         const double *src = (sa == S.u ? a.axr : a.axc) +
-                            ((((long long)tpos[S.xp] * a.nb + R0) * a.nb + R1) * 16 + sidx) * (16LL * a.kp);
+                            (((long long)tpos[S.xp] * a.nb + R0) * a.nb + R1) * (256LL * a.kp) +
+                            (long long)(sidx >> 1) * (32LL * a.kp) + (sidx & 1);
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int idx = 2 * (tid + k * GTHR);
-            const int Kl = idx >> 4;
-            double2 v = {0.0, 0.0};
-            if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(src + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
-            ra[2 * k] = v.x; ra[2 * k + 1] = v.y;
+        for (int k = 0; k < 4; ++k) {
+            const int e = tid + k * GTHR;                 // (K, o) of the 128 x 16 slice
+            const int Kl = e >> 4, o = e & 15, K = K0 + Kl;
+            ra[k] = K < a.kp ? src[((long long)(K >> 2) * 64 + (K & 3) * 16 + o) * 2] : 0.0;
         }
     };
     auto stash = [&](int buf) {
@@ -626,11 +627,9 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
         }
         double *As = AS + buf * GCH * BS_STR;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int idx = 2 * (tid + k * GTHR);
-            const int Kl = idx >> 4, o = idx & 15;
-            As[Kl * BS_STR + o] = ra[2 * k];
-            As[Kl * BS_STR + o + 1] = ra[2 * k + 1];
+        for (int k = 0; k < 4; ++k) {
+            const int e = tid + k * GTHR;
+            As[(e >> 4) * BS_STR + (e & 15)] = ra[k];
         }
     };
 

@@ -154,7 +154,7 @@ struct ItemCtx {
 // fragment of K step kb is base[kb*64 + lane].
 template <class T>
 struct PhasePtrs {
-    const T *p[6][BPW];   // operand slot, batch
+    const T *p[6];        // operand slot: the wave's two batches, interleaved per lane
     const T *x[6];        // BX slot
 };
 
@@ -175,8 +175,7 @@ __device__ __forceinline__ void phase_ptrs(const TriplesArgs &g, const ItemCtx &
         const int o = s / 3, xp = s % 3;
         const T *base = static_cast<const T *>(col[o] ? g.axc : g.axr) +
                         (((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o]) * per_panel;
-#pragma unroll
-        for (int bb = 0; bb < BPW; ++bb) pp.p[s][bb] = base + (long long)(wave * BPW + bb) * (16LL * g.kp);
+        pp.p[s] = base + (long long)wave * (32LL * g.kp);
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q)
@@ -189,16 +188,21 @@ struct Frags {
     T x[6];
 };
 
+static_assert(BPW == 2, "a wave's two batches share one 2-wide operand load");
+
 template <int PHASE, int MODE, class T>
 __device__ __forceinline__ void load_frags(const PhasePtrs<T> &pp, int kb, int lane, Frags<T> &f)
 {
+    typedef typename std::conditional<std::is_same<T, double>::value, double2, float2>::type T2;
     constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
     const int off = kb * 64 + lane;
 #pragma unroll
     for (int s = 0; s < 6; ++s)
-        if (SM & (1 << s))
-#pragma unroll
-            for (int bb = 0; bb < BPW; ++bb) f.p[s][bb] = pp.p[s][bb][off];
+        if (SM & (1 << s)) {
+            const T2 v = reinterpret_cast<const T2 *>(pp.p[s])[off];
+            f.p[s][0] = v.x;
+            f.p[s][1] = v.y;
+        }
 #pragma unroll
     for (int q = 0; q < 6; ++q)
         if (XM & (1 << q)) f.x[q] = pp.x[q][off];
@@ -242,25 +246,38 @@ __device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &i
 {
     PhasePtrs<T> pp;
     phase_ptrs<PHASE, T>(g, it, wave, pp);
+    // Ring of NBUF register buffers: the loads of step kb+NBUF-1 are in flight while step kb
+    // computes.  FP64 uses 2 (one step covers the load latency at the FP64 MFMA rate); FP32
+    // fragments take half the registers and its MFMAs run twice as fast, so it uses 4.
+    constexpr int NBUF = std::is_same<T, double>::value ? 2 : 4;
     const int nkb = g.kp / KS;
-    Frags<T> f0, f1;
-    load_frags<PHASE, MODE, T>(pp, 0, lane, f0);
-    // Every load is unconditional (the last prefetch re-reads step nkb-1), so the number of
-    // loads in flight is the same on every path and each step waits only for its own
-    // fragments: a conditional prefetch makes the wait counter drain completely.
-    // The scheduling barriers keep each prefetch group ahead of the MFMAs of the current
-    // step (the scheduler would otherwise sink loads next to their uses).
-    for (int kb = 0; kb + 2 <= nkb; kb += 2) {
-        load_frags<PHASE, MODE, T>(pp, kb + 1, lane, f1);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma_step<PHASE, MODE, T>(f0, acc);
-        __builtin_amdgcn_sched_barrier(0);
-        load_frags<PHASE, MODE, T>(pp, kb + 2 < nkb ? kb + 2 : nkb - 1, lane, f0);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma_step<PHASE, MODE, T>(f1, acc);
-        __builtin_amdgcn_sched_barrier(0);
+    const int last = nkb - 1;
+    Frags<T> f[NBUF];
+    static_for<NBUF - 1>([&](auto B) __attribute__((always_inline)) {
+        constexpr int b = decltype(B)::value;
+        load_frags<PHASE, MODE, T>(pp, b < last ? b : last, lane, f[b]);
+    });
+    // Every load is unconditional (prefetches past the end re-read step nkb-1), so the number
+    // of loads in flight is the same on every path and each step waits only for its own
+    // fragments (a conditional prefetch makes the wait counter drain completely).  The
+    // scheduling barriers keep each prefetch group ahead of the current step's MFMAs (the
+    // scheduler would otherwise sink loads next to their uses).
+    int kb = 0;
+    for (; kb + NBUF <= nkb; kb += NBUF) {
+        static_for<NBUF>([&](auto J) __attribute__((always_inline)) {
+            constexpr int jn = decltype(J)::value;
+            const int nxt = kb + jn + NBUF - 1;
+            load_frags<PHASE, MODE, T>(pp, nxt < last ? nxt : last, lane, f[(jn + NBUF - 1) % NBUF]);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_step<PHASE, MODE, T>(f[jn], acc);
+            __builtin_amdgcn_sched_barrier(0);
+        });
     }
-    if (nkb & 1) mfma_step<PHASE, MODE, T>(f0, acc);
+    // the remaining nkb - kb (< NBUF) steps are in f[0 ..]
+    static_for<NBUF - 1>([&](auto J) __attribute__((always_inline)) {
+        constexpr int jn = decltype(J)::value;
+        if (kb + jn < nkb) mfma_step<PHASE, MODE, T>(f[jn], acc);
+    });
 }
 
 __device__ __forceinline__ double wave_sum(double v)

@@ -57,9 +57,14 @@ __global__ void pack_ax_kernel(const double *__restrict__ vvvo, const double *__
          idx += (long long)gridDim.x * blockDim.x) {
         long long r = idx % per_panel;
         long long pnl = idx / per_panel;
-        const int inner = r & 15;                 // fastest: beta (rows) / alpha (columns)
-        const long long K = (r >> 4) % kp;        // kb*4 + k4
-        const int outer = (int)(r / (16LL * kp)); // alpha (rows) / beta (columns)
+        // [outer/2][kp/4][64 lanes][2]: the fragments of the two batches (outer indices 2w,
+        // 2w+1) a wave owns interleave per lane, so one 2-wide load fetches both
+        const int bbit = (int)(r & 1);
+        const int lane = (int)((r >> 1) & 63);
+        const long long rr = r >> 7;
+        const long long K = (rr % (kp / 4)) * 4 + (lane >> 4);   // kb*4 + k4
+        const int outer = (int)(rr / (kp / 4)) * 2 + bbit;       // alpha (rows) / beta (columns)
+        const int inner = lane & 15;                              // beta (rows) / alpha (columns)
         const int q = pnl % nb; pnl /= nb;
         const int pblk = pnl % nb; pnl /= nb;
         const int x = (int)pnl;

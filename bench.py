@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 METRIC = "(T) time-to-solution & matrix-peak %; split-prec vs FP64 at |ΔE(T)|<1 µEh"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+# executed MFMA FLOPs per algorithmic FLOP: the split BF16 form runs 6 slice products
+FORM_PEAK = {"fp64": (PEAK_FP64_TFLOPS, 1), "fp32": (PEAK_FP32_TFLOPS, 1), "bf16x3": (PEAK_BF16_TFLOPS, 6)}
 CONFIGS = {"c1": (4, 36, 1001), "c2": (4, 87, 1002), "c3": (8, 120, 1003), "c4": (24, 216, 1004),
            "c5": (30, 400, 1005)}
 
@@ -120,7 +123,7 @@ def main() -> None:
                     help="only summarize rocprofv3 result databases as markdown")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run one adjoint (gradient) evaluation and print its timing")
-    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32"],
+    ap.add_argument("--form", default="fp64", choices=sorted(FORM_PEAK),
                     help="arithmetic form of the timed steps (the headline metric is fp64)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other arithmetic forms and the adjoint in the JSON line")
@@ -183,7 +186,7 @@ def main() -> None:
     items, exec_flops = ctx.work_stats()
     n_tup = no * (no - 1) * (no - 2) // 6 + no * (no - 1)
     achieved = flops / kernel_s / 1e12
-    peak = PEAK_FP64_TFLOPS if args.form == "fp64" else PEAK_FP32_TFLOPS
+    peak = FORM_PEAK[args.form][0]
     out = {
         "metric": METRIC,
         "value": ms_step / 1e3,
@@ -195,7 +198,7 @@ def main() -> None:
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32acc"}[args.form],
         "data": "synthetic",
         "config": {"workload": f"closed-shell (T) E(T)=E[T]+E_ST and v1, no={no} nu={nu}, "
                                f"{no * (no - 1) * (no - 2) // 6} distinct + {no * (no - 1)} degenerate tuples, "
@@ -215,13 +218,17 @@ def main() -> None:
         out["forms"] = {"fp64": {"kernel_ms": kernel_s * 1e3, "e_t": r["e"], "abs_de": 0.0,
                                  "frac_of_peak": achieved / PEAK_FP64_TFLOPS,
                                  "peak_tflops": PEAK_FP64_TFLOPS}}
-        for form, peak in (("fp32", PEAK_FP32_TFLOPS),):
+        for form in ("fp32", "bf16x3"):
+            peak, mult = FORM_PEAK[form]
+            log(form)
             ctx.energy(form)
             rf = ctx.energy(form)
             kf = ctx.timing()[0] / 1e3
             out["forms"][form] = {"kernel_ms": kf * 1e3, "e_t": rf["e"], "abs_de": abs(rf["e"] - r["e"]),
+                                  "rel_de": abs(rf["e"] - r["e"]) / abs(r["e"]),
                                   "speedup_vs_fp64": kernel_s / kf,
-                                  "frac_of_peak": flops / kf / 1e12 / peak, "peak_tflops": peak}
+                                  "frac_of_peak": mult * flops / kf / 1e12 / peak, "peak_tflops": peak,
+                                  "executed_flops_per_algorithmic": mult}
         # the adjoint: dE/d(T1, T2, VM, VOE, VVVO) with per-item recomputation of W
         log("adjoint")
         g = ctx.gradient(device=True)

@@ -21,6 +21,11 @@ struct ccx_ctx {
     double *d_axr, *d_axc, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
     float *d_axr32, *d_axc32, *d_bx32; /* FP32 form slabs, converted on first use after a load */
     int f32_ready;
+    void *d_axr_s, *d_axc_s, *d_bx_s;  /* split BF16 form: 3 slice planes each, built on first use */
+    unsigned long long *d_max;
+    int64_t kps;                        /* K padded to 64 for the split form */
+    double split_scale;
+    int split_ready;
     int *d_tuples, *d_blocks;
     unsigned int *d_counter;
     double *d_epart, *d_v1part, *d_v1t, *d_v1, *d_out;
@@ -103,10 +108,17 @@ static void free_dev(ccx_ctx *c)
     ccx_hip_free(c->d_axr32);
     ccx_hip_free(c->d_axc32);
     ccx_hip_free(c->d_bx32);
+    ccx_hip_free(c->d_axr_s);
+    ccx_hip_free(c->d_axc_s);
+    ccx_hip_free(c->d_bx_s);
+    ccx_hip_free(c->d_max);
     c->d_tuples = c->d_blocks = NULL;
     c->d_counter = NULL;
     c->d_axr32 = c->d_axc32 = c->d_bx32 = NULL;
+    c->d_axr_s = c->d_axc_s = c->d_bx_s = NULL;
+    c->d_max = NULL;
     c->f32_ready = 0;
+    c->split_ready = 0;
     c->loaded = 0;
 }
 
@@ -124,6 +136,7 @@ int ccx_create(int64_t no, int64_t nu, ccx_ctx **out)
     c->nup = (nu + TB - 1) / TB * TB;
     c->nb = c->nup / TB;
     c->kp = (nu + no + 3) / 4 * 4;
+    c->kps = (nu + no + 63) / 64 * 64;
     c->ntup = build_tuples(no, NULL);
     c->nblk = build_blocks(c->nb, NULL);
     c->h_tuples = (int *)malloc(sizeof(int) * 4 * (size_t)(c->ntup > 0 ? c->ntup : 1));
@@ -198,6 +211,7 @@ static int build_from_device(ccx_ctx *c, const double *d_eh, const double *d_ep_
 allocated:
     c->loaded = 0;
     c->f32_ready = 0;
+    c->split_ready = 0;
     if (ccx_hip_pack(0, d_vvvo_full, d_t2, d_vm, d_voe, c->d_axr, c->d_axc, c->d_bx, c->d_voep, (int)no, (int)nu,
                      (int)nup, (int)kp, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
     /* eps_virtual padded to nup with finite values (padded elements carry W = 0) */
@@ -306,17 +320,46 @@ static int ensure_f32(ccx_ctx *c)
     return CCX_OK;
 }
 
+/* split BF16 slabs (3 bf16 slice planes of each FP64 slab), allocated on first use */
+static int ensure_split(ccx_ctx *c)
+{
+    if (c->split_ready) return CCX_OK;
+    const int64_t ax_slice = c->no * c->nb * c->nb * 256 * c->kps, bx_slice = c->no * c->no * c->nb * 16 * c->kps;
+    if (!c->d_axr_s) {
+        if (ccx_hip_malloc(&c->d_axr_s, 2 * 3 * (size_t)ax_slice) != 0 ||
+            ccx_hip_malloc(&c->d_axc_s, 2 * 3 * (size_t)ax_slice) != 0 ||
+            ccx_hip_malloc(&c->d_bx_s, 2 * 3 * (size_t)bx_slice) != 0 ||
+            ccx_hip_malloc((void **)&c->d_max, 2 * sizeof(unsigned long long)) != 0) {
+            ccx_hip_free(c->d_axr_s); ccx_hip_free(c->d_axc_s); ccx_hip_free(c->d_bx_s); ccx_hip_free(c->d_max);
+            c->d_axr_s = c->d_axc_s = c->d_bx_s = NULL;
+            c->d_max = NULL;
+            return CCX_ERR_ALLOC;
+        }
+    }
+    if (ccx_hip_pack_split(c->d_axr, c->d_axc, c->d_bx, c->d_axr_s, c->d_axc_s, c->d_bx_s, c->d_max, (int)c->no,
+                           (int)c->nb, (int)c->kp, (int)c->kps, &c->split_scale, c->stream) != 0)
+        return CCX_ERR_DEVICE;
+    c->split_ready = 1;
+    return CCX_OK;
+}
+
 int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v1_on_device)
 {
     if (!c) return CCX_ERR_ARG;
     if (!c->loaded) return CCX_ERR_STATE;
-    if (form != CCX_FORM_FP64 && form != CCX_FORM_FP32) return CCX_ERR_FORM;
+    if (form != CCX_FORM_FP64 && form != CCX_FORM_FP32 && form != CCX_FORM_BF16X3) return CCX_ERR_FORM;
     const int64_t nitems = c->ntup * c->nblk;
     double host_out[2] = {0.0, 0.0};
     if (form == CCX_FORM_FP32) {
         int rc = ensure_f32(c);
         if (rc != CCX_OK) return rc;
     }
+    if (form == CCX_FORM_BF16X3) {
+        int rc = ensure_split(c);
+        if (rc != CCX_OK) return rc;
+    }
+    const void *split[3] = {c->d_axr_s, c->d_axc_s, c->d_bx_s};
+    const int64_t ax_slice = c->no * c->nb * c->nb * 256 * c->kps, bx_slice = c->no * c->no * c->nb * 16 * c->kps;
     const void *axr = form == CCX_FORM_FP32 ? (const void *)c->d_axr32 : (const void *)c->d_axr;
     const void *axc = form == CCX_FORM_FP32 ? (const void *)c->d_axc32 : (const void *)c->d_axc;
     const void *bx = form == CCX_FORM_FP32 ? (const void *)c->d_bx32 : (const void *)c->d_bx;
@@ -325,7 +368,8 @@ int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v
     if (nitems > 0 &&
         ccx_hip_triples(form, axr, axc, bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
                         c->d_v1part, NULL, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp,
-                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), c->stream) != 0)
+                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), split, ax_slice, bx_slice, (int)c->kps,
+                        c->split_scale, c->stream) != 0)
         return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[2], c->stream) != 0) return CCX_ERR_DEVICE;
     if (ccx_hip_finalize(c->d_epart, c->d_v1part, c->d_tuples, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,
@@ -366,7 +410,7 @@ int ccx_tuple_w(ccx_ctx *c, int64_t i, int64_t j, int64_t k, double *w)
     if (ccx_hip_memcpy_h2d(d_tup, tup, sizeof(tup), c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
     if (ccx_hip_triples(0, c->d_axr, c->d_axc, c->d_bx, c->d_voep, c->d_eh, c->d_ep, d_tup, c->d_blocks, d_ep1,
                         d_vp, d_w, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, 1, (int)c->nblk,
-                        grid_size(c, c->nblk), c->stream) != 0 ||
+                        grid_size(c, c->nblk), NULL, 0, 0, 0, 1.0, c->stream) != 0 ||
         ccx_hip_memcpy_d2h(w, d_w, sizeof(double) * nu3, c->stream) != 0)
         rc = CCX_ERR_DEVICE;
 fail:

@@ -125,8 +125,8 @@ template <> struct Mfma<float> {
 }  // namespace
 
 struct TriplesArgs {
-    const void *axr;       // [no][nb][nb][16 alpha][kp/4][4][16 beta]  (row fragments), type T
-    const void *axc;       // [no][nb][nb][16 beta][kp/4][4][16 alpha]  (column fragments), type T
+    const void *axr;       // [no][nb][nb][8 alpha pairs][kp/4][4*16 lanes: k4, beta][2 alpha] (row), type T
+    const void *axc;       // [no][nb][nb][8 beta pairs][kp/4][4*16 lanes: k4, alpha][2 beta] (column), type T
     const void *bx;        // [no][no][nb][kp/4][4][16 gamma], type T
     const double *voe;     // [no][no][nup][nup], voe[q][r][x][y] = VOE(x,y,q,r)
     const double *eh;      // [no]
@@ -140,6 +140,13 @@ struct TriplesArgs {
     int no, nu, nup, nb, kp, ntup, nblk;
     int block_major;       // work order: 1 block triple major, 0 tuple major
     long long nitems;
+    // split BF16 form: 3 fixed-point bf16 slices of AX (both orientations) and BX in
+    // v_mfma_f32_16x16x32_bf16 fragment order, K padded to kps (a multiple of 64);
+    // W = (FP32 accumulation of the slice products) * wscale
+    const void *axr_s, *axc_s, *bx_s;
+    long long ax_slice, bx_slice;   // elements per slice plane
+    int kps;
+    double wscale;
 };
 
 namespace {

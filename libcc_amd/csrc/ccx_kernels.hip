@@ -32,6 +32,7 @@
 
 #include "ccx_item.h"
 #include <stdlib.h>
+#include <string.h>
 
 
 // ---------------------------------------------------------------------------------
@@ -134,16 +135,230 @@ __global__ void unpack_vvvo_kernel(const double *__restrict__ packed, double *__
 }
 
 // ---------------------------------------------------------------------------------
+// split BF16 form: fixed-point slices of the FP64 operand slabs
+// ---------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// max |x| over a slab (non-negative doubles order like their bit patterns)
+__global__ void absmax_kernel(const double *__restrict__ x, long long n, unsigned long long *__restrict__ out)
+{
+    double m = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// x in [-1, 1] -> x0 + x1 + x2 with x_i = n_i 2^(-7-8i), |n_i| <= 128: exact in bf16, so every
+// slice product n_i m_j 2^(-14-8(i+j)) is exact in FP32; the remainder is below 2^-24.
+__device__ __forceinline__ void split3(double x, __bf16 &a0, __bf16 &a1, __bf16 &a2)
+{
+    const double n0 = rint(x * 128.0);
+    double r = x - n0 * (1.0 / 128.0);
+    const double n1 = rint(r * 32768.0);
+    r -= n1 * (1.0 / 32768.0);
+    const double n2 = rint(r * 8388608.0);
+    a0 = (__bf16)(float)(n0 * (1.0 / 128.0));
+    a1 = (__bf16)(float)(n1 * (1.0 / 32768.0));
+    a2 = (__bf16)(float)(n2 * (1.0 / 8388608.0));
+}
+
+// Slice planes in v_mfma_f32_16x16x32_bf16 fragment order: panel, outer index (batch), K block
+// of 32, lane (inner index = lane & 15, K = 32 kb + 8 (lane >> 4) + e), e.  BX = 1: the source
+// is the BX slab [panel][K][16 gamma] and there is no outer index.
+template <int BX>
+__global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__restrict__ dst, long long npanel,
+                                  int kp, int kps, double inv_scale, long long slice)
+{
+    const int nkb = kps / 32;
+    const long long per_panel = (BX ? 16LL : 256LL) * kps;
+    const long long total = npanel * per_panel;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(idx & 7), lane = (int)((idx >> 3) & 63);
+        long long rr = idx >> 9;
+        const int kb = (int)(rr % nkb);
+        rr /= nkb;
+        const int K = kb * 32 + (lane >> 4) * 8 + e, inner = lane & 15;
+        double v = 0.0;
+        if (K < kp) {
+            if (BX) {
+                v = src[rr * (16LL * kp) + (long long)K * 16 + inner];
+            } else {
+                const int outer = (int)(rr % 16);
+                const long long panel = rr / 16;
+                v = src[panel * (256LL * kp) + (((long long)(outer >> 1) * (kp / 4) + K / 4) * 64 + (K & 3) * 16 + inner) * 2 +
+                        (outer & 1)];
+            }
+        }
+        __bf16 a0, a1, a2;
+        split3(v * inv_scale, a0, a1, a2);
+        dst[idx] = a0;
+        dst[idx + slice] = a1;
+        dst[idx + 2 * slice] = a2;
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // persistent (tuple, block-orbit) kernel, FP64 MFMA
 // ---------------------------------------------------------------------------------
 namespace {
 
-template <int MODE, class T>
+// ---- split BF16 K sweep -----------------------------------------------------------
+// n-th set bit of a mask (compile time)
+__host__ __device__ constexpr int nth_bit(int mask, int n)
+{
+    for (int b = 0; b < 32; ++b)
+        if ((mask >> b) & 1) {
+            if (n == 0) return b;
+            --n;
+        }
+    return -1;
+}
+
+// Blocks of 32 K: the BX slices of the phase (shared by all waves) are staged in LDS, double
+// buffered across blocks; the AX slices stream per operand slot through a two-deep register
+// ring (two blocks per loop iteration keep the ring index a compile-time constant).  Each
+// slot's MFMAs are the 6 slice products i + j <= 2 of its terms, FP32 accumulation.
+template <int PHASE, int MODE>
+__device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const ItemCtx &it, f4 (&acc)[6][BPW],
+                                                __bf16 *lbx, int wave, int lane)
+{
+    constexpr int MM = mode_mask(MODE);
+    constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
+    constexpr int NSL = __builtin_popcount(SM), NBX = __builtin_popcount(XM);
+    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
+    int r0[2], r1[2];
+    bool col[2];
+    if (PHASE == 0) { r0[0] = A; r1[0] = C; col[0] = true;  r0[1] = C; r1[1] = A; col[1] = false; }
+    else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
+    else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
+    const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
+    const long long kps = g.kps;
+    const __bf16 *ap[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int o = s / 3, xp = s % 3;
+        ap[s] = static_cast<const __bf16 *>(col[o] ? g.axc_s : g.axr_s) +
+                (((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o]) * (256LL * kps) + (long long)wave * (32LL * kps);
+    }
+    const __bf16 *xq[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        xq[q] = static_cast<const __bf16 *>(g.bx_s) + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (16LL * kps);
+    const int nkb = g.kps / 32;
+    const int lo = lane * 8;
+
+    // BX staging: chunk c = rank*3 + slice (64 lanes x 16 B) is loaded by wave c % NWAVE
+    bf16x8 br[3];
+    auto bx_load = [&](int kb) __attribute__((always_inline)) {
+        static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(Cc)::value;
+            constexpr int q = nth_bit(XM, c / 3), sl = c % 3;
+            if (wave == c % NWAVE)
+                br[c / NWAVE] = *reinterpret_cast<const bf16x8 *>(xq[q] + sl * g.bx_slice + (long long)kb * 512 + lo);
+        });
+    };
+    auto bx_store = [&](int buf) __attribute__((always_inline)) {
+        static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(Cc)::value;
+            if (wave == c % NWAVE) *reinterpret_cast<bf16x8 *>(lbx + ((buf * NBX) * 3 + c) * 512 + lo) = br[c / NWAVE];
+        });
+    };
+    bf16x8 ab[2][BPW][3];
+    auto a_load = [&](int s, int kb, bf16x8 (&d)[BPW][3]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+            for (int sl = 0; sl < 3; ++sl)
+                d[bb][sl] = *reinterpret_cast<const bf16x8 *>(ap[s] + (long long)bb * (16LL * kps) + sl * g.ax_slice +
+                                                              (long long)kb * 512 + lo);
+    };
+    auto compute = [&](auto Sc, const bf16x8 (&a)[BPW][3], int buf) __attribute__((always_inline)) {
+        constexpr int s = decltype(Sc)::value;
+        static_for<6>([&](auto P) __attribute__((always_inline)) {
+            constexpr int p = decltype(P)::value;
+            if constexpr ((MM >> p) & 1) {
+                constexpr Term t = term(PHASE, p);
+                static_for<2>([&](auto U) __attribute__((always_inline)) {
+                    constexpr int u = decltype(U)::value;
+                    constexpr int ts = u == 0 ? t.s1 : t.s2, tb = u == 0 ? t.b1 : t.b2;
+                    if constexpr (ts == s) {
+                        constexpr int rank = __builtin_popcount(XM & ((1 << tb) - 1));
+                        bf16x8 bs[3];
+#pragma unroll
+                        for (int sl = 0; sl < 3; ++sl)
+                            bs[sl] = *reinterpret_cast<const bf16x8 *>(lbx + ((buf * NBX + rank) * 3 + sl) * 512 + lo);
+#pragma unroll
+                        for (int bb = 0; bb < BPW; ++bb) {
+                            f4 c = acc[p][bb];
+                            // slice products with i + j <= 2, smallest first
+                            static_for<6>([&](auto Q) __attribute__((always_inline)) {
+                                constexpr int qq = 5 - decltype(Q)::value;
+                                constexpr int si = qq == 0 ? 0 : qq == 1 ? 0 : qq == 2 ? 1 : qq == 3 ? 0 : qq == 4 ? 1 : 2;
+                                constexpr int sj = qq == 0 ? 0 : qq == 1 ? 1 : qq == 2 ? 0 : qq == 3 ? 2 : qq == 4 ? 1 : 0;
+                                if constexpr (PHASE == 2)
+                                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bs[sj], a[bb][si], c, 0, 0, 0);
+                                else
+                                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[bb][si], bs[sj], c, 0, 0, 0);
+                            });
+                            acc[p][bb] = c;
+                        }
+                    }
+                });
+            }
+        });
+    };
+
+    bx_load(0);
+    bx_store(0);
+    __syncthreads();
+    a_load(nth_bit(SM, 0), 0, ab[0]);
+    for (int kb = 0; kb < nkb; kb += 2) {
+        static_for<2>([&](auto HB) __attribute__((always_inline)) {
+            constexpr int hb = decltype(HB)::value;
+            const int kbb = kb + hb;
+            bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+            static_for<NSL>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                constexpr int t = hb * NSL + j;
+                constexpr int s = nth_bit(SM, j);
+                constexpr int sn = nth_bit(SM, (j + 1) % NSL);
+                const int kn = j + 1 < NSL ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+                a_load(sn, kn, ab[(t + 1) % 2]);
+                __builtin_amdgcn_sched_barrier(0);
+                compute(std::integral_constant<int, s>{}, ab[t % 2], hb);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            bx_store(1 - hb);
+            __syncthreads();
+        });
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void item_w_split(const TriplesArgs &g, double *lds, const ItemCtx &it,
+                                             f4 (&acc)[6][BPW], int wave, int lane)
+{
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) acc[p][bb] = f4{0, 0, 0, 0};
+    __bf16 *lbx = reinterpret_cast<__bf16 *>(lds);
+    run_phase_split<0, MODE>(g, it, acc, lbx, wave, lane);
+    transpose_lb_to_l<MODE, float>(lds, acc, wave, lane);
+    run_phase_split<1, MODE>(g, it, acc, lbx, wave, lane);
+    run_phase_split<2, MODE>(g, it, acc, lbx, wave, lane);
+    complete_orderings<MODE, float>(lds, acc, wave, lane);
+}
+
+template <int MODE, class T, bool SPLIT>
 __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
                         int tkind, int mult, int wave, int lane, int tid)
 {
     typename Mfma<T>::acc_t acc[6][BPW];
-    item_w<MODE, T>(g, lds, it, acc, wave, lane);
+    if constexpr (SPLIT) item_w_split<MODE>(g, lds, it, acc, wave, lane);
+    else item_w<MODE, T>(g, lds, it, acc, wave, lane);
 
     const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
     // ---- stage the 36 VOE sub-blocks: [ordered tuple-position pair][ordered block pair]
@@ -186,7 +401,10 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
             if (ag >= g.nu || bg >= g.nu || cg >= g.nu) continue;
             double w[6];
 #pragma unroll
-            for (int p = 0; p < 6; ++p) w[p] = (double)acc[p][bb][v];
+            for (int p = 0; p < 6; ++p) {
+                w[p] = (double)acc[p][bb][v];
+                if constexpr (SPLIT) w[p] *= g.wscale;
+            }
             const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
             const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
             const double sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3] + w[4] * w[4] + w[5] * w[5];
@@ -276,7 +494,7 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
 
 }  // namespace
 
-template <class T>
+template <class T, bool SPLIT>
 __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
 {
     extern __shared__ double lds[];
@@ -304,12 +522,12 @@ __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
         const long long item = (long long)t * g.nblk + q;   // storage index
         int tkind, mult;
         const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
-        if (tkind == 2) do_item<M_IJJ, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (tkind == 3) do_item<M_IIJ, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (mult == 1) do_item<M_FULL, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (mult == 6) do_item<M_AAA, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else do_item<M_ACC, T>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        if (tkind == 2) do_item<M_IJJ, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (tkind == 3) do_item<M_IIJ, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (mult == 1) do_item<M_FULL, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (mult == 6) do_item<M_AAA, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else do_item<M_ACC, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
     }
 }
 
@@ -529,16 +747,20 @@ int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *v
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// form 0: FP64 slabs; 1: FP32 slabs; 2: split BF16 (split[] = axr_s, axc_s, bx_s, W scaled by wscale)
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
-                    int ntup, int nblk, int grid, void *stream)
+                    int ntup, int nblk, int grid, const void *const *split, long long ax_slice, long long bx_slice,
+                    int kps, double wscale, void *stream)
 {
-    static bool attr_set[2] = {false, false};
+    static bool attr_set[3] = {false, false, false};
     const size_t lds_bytes = sizeof(double) * LDS_DOUBLES;
-    if (form != 0 && form != 1) return 3;
+    if (form < 0 || form > 2) return 3;
+    if (form == 2 && (split == nullptr || kps % 64 != 0)) return 3;
     if (!attr_set[form]) {
-        const void *fn = form == 0 ? (const void *)triples_kernel<double> : (const void *)triples_kernel<float>;
+        const void *fn = form == 0 ? (const void *)triples_kernel<double, false>
+                       : form == 1 ? (const void *)triples_kernel<float, false> : (const void *)triples_kernel<float, true>;
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return 2;
         attr_set[form] = true;
@@ -548,6 +770,11 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     g.blocks = blocks; g.e_part = e_part; g.v1_part = v1_part; g.wout = wout; g.counter = counter;
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
+    g.axr_s = form == 2 ? split[0] : nullptr;
+    g.axc_s = form == 2 ? split[1] : nullptr;
+    g.bx_s = form == 2 ? split[2] : nullptr;
+    g.ax_slice = ax_slice; g.bx_slice = bx_slice; g.kps = kps;
+    g.wscale = form == 2 ? wscale : 1.0;
     {
         const char *env = getenv("CCX_ITEM_ORDER");
         g.block_major = (env && env[0] == 't') ? 0 : 1;
@@ -557,8 +784,41 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     if (grid > g.nitems) grid = (int)g.nitems;
     hipStream_t s = (hipStream_t)stream;
     zero_counter_kernel<<<1, 1, 0, s>>>(counter);
-    if (form == 0) triples_kernel<double><<<grid, NTHR, lds_bytes, s>>>(g);
-    else triples_kernel<float><<<grid, NTHR, lds_bytes, s>>>(g);
+    if (form == 0) triples_kernel<double, false><<<grid, NTHR, lds_bytes, s>>>(g);
+    else if (form == 1) triples_kernel<float, false><<<grid, NTHR, lds_bytes, s>>>(g);
+    else triples_kernel<float, true><<<grid, NTHR, lds_bytes, s>>>(g);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Split BF16 slabs from the FP64 ones: power-of-two scales sA >= max|AX|, sB >= max|BX|, three
+// fixed-point bf16 slices of AX/sA (both orientations) and BX/sB; *wscale = sA * sB.
+int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, void *axr_s, void *axc_s, void *bx_s,
+                       unsigned long long *dmax, int no, int nb, int kp, int kps, double *wscale, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const long long n_ax = (long long)no * nb * nb * 256 * kp, n_bx = (long long)no * no * nb * 16 * kp;
+    if (hipMemsetAsync(dmax, 0, 2 * sizeof(unsigned long long), s) != hipSuccess) return 1;
+    absmax_kernel<<<grid_for(n_ax), 256, 0, s>>>(axr, n_ax, dmax);
+    absmax_kernel<<<grid_for(n_bx), 256, 0, s>>>(bx, n_bx, dmax + 1);
+    unsigned long long h[2];
+    if (hipMemcpyAsync(h, dmax, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return 1;
+    double sc[2];
+    for (int q = 0; q < 2; ++q) {
+        double m;
+        memcpy(&m, &h[q], sizeof(m));
+        int e = 0;
+        if (m > 0.0) frexp(m, &e);   // m < 2^e
+        sc[q] = m > 0.0 ? ldexp(1.0, e) : 1.0;
+    }
+    const long long ax_slice = (long long)no * nb * nb * 256 * kps, bx_slice = (long long)no * no * nb * 16 * kps;
+    pack_split_kernel<0><<<grid_for(ax_slice), 256, 0, s>>>(axr, (__bf16 *)axr_s, (long long)no * nb * nb, kp, kps,
+                                                             1.0 / sc[0], ax_slice);
+    pack_split_kernel<0><<<grid_for(ax_slice), 256, 0, s>>>(axc, (__bf16 *)axc_s, (long long)no * nb * nb, kp, kps,
+                                                             1.0 / sc[0], ax_slice);
+    pack_split_kernel<1><<<grid_for(bx_slice), 256, 0, s>>>(bx, (__bf16 *)bx_s, (long long)no * no * nb, kp, kps,
+                                                             1.0 / sc[1], bx_slice);
+    *wscale = sc[0] * sc[1];
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

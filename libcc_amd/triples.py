@@ -16,7 +16,7 @@ import numpy as np
 
 from . import CcxError, lib
 
-FORMS = {"fp64": 0, "fp32": 1}
+FORMS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 _dp = ctypes.POINTER(ctypes.c_double)
 
 

@@ -1,0 +1,86 @@
+"""The reduced-precision arithmetic forms against the FP64 form of the same library.
+
+FP32 (FP32 operands, FP32 MFMA accumulation) and the split BF16 form (three fixed-point
+bf16 slices per FP64 operand under one power-of-two scale per operand slab, six slice
+products accumulated in FP32, W rescaled in FP64) both keep the denominators, the epilogue
+and every reduction in FP64.  Their error against FP64 is that of FP32 accumulation:
+the bar here is 1e-6 relative on E[T] and E_ST and 1e-5 of max|v1| on v1 (v1 sums W
+against T1-sized factors with cancellation, so it keeps fewer correct digits)."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from libcc_amd import lib, synth
+from libcc_amd.triples import TriplesContext
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ("fp32", "bf16x3")
+
+
+def _close(got, want, rtol, v1_rtol=1e-5):
+    # absolute floor 1e-12 Eh: at nu=1 every W element is a diagonal one and the energies are
+    # FP64 round-off (1e-23), so there is no relative scale to compare against
+    tol = max(rtol * max(abs(want["etd"]), abs(want["ets"])), 1e-12)
+    assert abs(got["etd"] - want["etd"]) <= tol, (got["etd"], want["etd"])
+    assert abs(got["ets"] - want["ets"]) <= tol, (got["ets"], want["ets"])
+    vtol = max(v1_rtol * np.abs(want["v1"]).max(), 1e-12)
+    assert np.abs(got["v1"] - want["v1"]).max() <= vtol
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("no,nu,seed", [(4, 36, 1001), (3, 17, 6), (7, 20, 12), (8, 120, 1003), (5, 1, 2),
+                                        (2, 33, 4)], ids=lambda v: str(v))
+def test_form_matches_fp64(form, no, nu, seed):
+    """Configurations with ragged 16-blocks (17, 20, 33), nu=1, degenerate tuples only (no=2)."""
+    ctx = TriplesContext(no, nu)
+    ctx.load(synth.make_inputs(no, nu, seed))
+    want = ctx.energy("fp64")
+    got = ctx.energy(form)
+    _close(got, want, 1e-6)
+    # run to run bitwise
+    again = ctx.energy(form)
+    assert again["etd"] == got["etd"] and again["ets"] == got["ets"]
+    assert np.array_equal(again["v1"], got["v1"])
+    ctx.close()
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_form_follows_reload(form):
+    """The converted operand slabs are rebuilt after every load: a second input set with
+    VVVO and VM scaled by 3 (not a power of two, so the split scales change) gives that
+    input's energy, not a stale one."""
+    no, nu, seed = 6, 40, 77
+    inp = synth.make_inputs(no, nu, seed)
+    ctx = TriplesContext(no, nu)
+    ctx.load(inp)
+    first = ctx.energy(form)
+    sc = dict(inp, vvvo=np.asfortranarray(inp["vvvo"] * 3.0), vm=np.asfortranarray(inp["vm"] * 3.0))
+    ctx.load(sc)
+    want = ctx.energy("fp64")
+    got = ctx.energy(form)
+    _close(got, want, 1e-6)
+    assert abs(got["etd"] - 9.0 * first["etd"]) <= 1e-5 * abs(got["etd"])
+    ctx.close()
+
+
+def test_no_tuples_is_zero():
+    ctx = TriplesContext(1, 20)
+    ctx.load(synth.make_inputs(1, 20, 3))
+    for form in FORMS:
+        r = ctx.energy(form)
+        assert r["etd"] == 0.0 and r["ets"] == 0.0
+    ctx.close()
+
+
+def test_unknown_form_rejected():
+    ctx = TriplesContext(3, 5)
+    ctx.load(synth.make_inputs(3, 5, 7))
+    ets, etd = ctypes.c_double(), ctypes.c_double()
+    st = lib().ccx_energy(ctx._ctx, 7, ctypes.byref(ets), ctypes.byref(etd), None, 0)
+    assert st != 0
+    with pytest.raises(KeyError):
+        ctx.energy("fp8x4")
+    ctx.close()

@@ -142,8 +142,8 @@ def main() -> None:
         print(json.dumps({"copy_probe_bytes": n * 8}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this path's tuples would shard over GPUs with one final "
-              f"reduction (see DESIGN.md), but no multi-GPU run is part of this build (--gpus {args.gpus}).",
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the tuples would shard "
+              f"over GPUs with one final energy reduction, but no multi-GPU path is implemented.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -43,9 +43,10 @@ extern "C" {
  * reductions are FP64 in every form */
 #define CCX_FORM_FP64 0    /* full FP64 (v_mfma_f64_16x16x4_f64)              */
 #define CCX_FORM_FP32 1    /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32) */
-#define CCX_FORM_BF16X3 2  /* split precision: each FP64 operand as 3 fixed-point bf16 slices under a
-                            * power-of-two scale, the 6 slice products with i+j <= 2 on
-                            * v_mfma_f32_16x16x32_bf16 with FP32 accumulation, FP64 scaling */
+#define CCX_FORM_BF16X3 2  /* split precision: each FP64 operand as hi + mid + lo bf16 slices (about
+                            * 26 bits of each element, under one power-of-two scale per slab), the 6
+                            * slice products with i+j <= 2 on v_mfma_f32_16x16x32_bf16 with FP32
+                            * accumulation; W, the epilogue and all reductions in FP64 */
 
 const char *ccx_error_string(int status);
 int ccx_last_status(void);          /* status of the last legacy (void) call    */

@@ -23,8 +23,7 @@ int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *v
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
-                    int ntup, int nblk, int grid, const void *const *split, long long ax_slice, long long bx_slice,
-                    int kps, double wscale, void *stream);
+                    int ntup, int nblk, int grid, const void *const *split, int kps, double wscale, void *stream);
 int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, void *axr_s, void *axc_s, void *bx_s,
                        unsigned long long *dmax, int no, int nb, int kp, int kps, double *wscale, void *stream);
 int ccx_hip_to_f32(const double *src, float *dst, long long n, void *stream);

@@ -21,7 +21,7 @@ struct ccx_ctx {
     double *d_axr, *d_axc, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
     float *d_axr32, *d_axc32, *d_bx32; /* FP32 form slabs, converted on first use after a load */
     int f32_ready;
-    void *d_axr_s, *d_axc_s, *d_bx_s;  /* split BF16 form: 3 slice planes each, built on first use */
+    void *d_axr_s, *d_axc_s, *d_bx_s;  /* split BF16 form: 3 bf16 slices per element, built on first use */
     unsigned long long *d_max;
     int64_t kps;                        /* K padded to 64 for the split form */
     double split_scale;
@@ -359,7 +359,6 @@ int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v
         if (rc != CCX_OK) return rc;
     }
     const void *split[3] = {c->d_axr_s, c->d_axc_s, c->d_bx_s};
-    const int64_t ax_slice = c->no * c->nb * c->nb * 256 * c->kps, bx_slice = c->no * c->no * c->nb * 16 * c->kps;
     const void *axr = form == CCX_FORM_FP32 ? (const void *)c->d_axr32 : (const void *)c->d_axr;
     const void *axc = form == CCX_FORM_FP32 ? (const void *)c->d_axc32 : (const void *)c->d_axc;
     const void *bx = form == CCX_FORM_FP32 ? (const void *)c->d_bx32 : (const void *)c->d_bx;
@@ -368,7 +367,7 @@ int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v
     if (nitems > 0 &&
         ccx_hip_triples(form, axr, axc, bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
                         c->d_v1part, NULL, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp,
-                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), split, ax_slice, bx_slice, (int)c->kps,
+                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), split, (int)c->kps,
                         c->split_scale, c->stream) != 0)
         return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[2], c->stream) != 0) return CCX_ERR_DEVICE;
@@ -410,7 +409,7 @@ int ccx_tuple_w(ccx_ctx *c, int64_t i, int64_t j, int64_t k, double *w)
     if (ccx_hip_memcpy_h2d(d_tup, tup, sizeof(tup), c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
     if (ccx_hip_triples(0, c->d_axr, c->d_axc, c->d_bx, c->d_voep, c->d_eh, c->d_ep, d_tup, c->d_blocks, d_ep1,
                         d_vp, d_w, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp, 1, (int)c->nblk,
-                        grid_size(c, c->nblk), NULL, 0, 0, 0, 1.0, c->stream) != 0 ||
+                        grid_size(c, c->nblk), NULL, 0, 1.0, c->stream) != 0 ||
         ccx_hip_memcpy_d2h(w, d_w, sizeof(double) * nu3, c->stream) != 0)
         rc = CCX_ERR_DEVICE;
 fail:

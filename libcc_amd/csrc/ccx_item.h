@@ -140,11 +140,10 @@ struct TriplesArgs {
     int no, nu, nup, nb, kp, ntup, nblk;
     int block_major;       // work order: 1 block triple major, 0 tuple major
     long long nitems;
-    // split BF16 form: 3 fixed-point bf16 slices of AX (both orientations) and BX in
+    // split BF16 form: hi/mid/lo bf16 slices of AX (both orientations) and BX in
     // v_mfma_f32_16x16x32_bf16 fragment order, K padded to kps (a multiple of 64);
     // W = (FP32 accumulation of the slice products) * wscale
     const void *axr_s, *axc_s, *bx_s;
-    long long ax_slice, bx_slice;   // elements per slice plane
     int kps;
     double wscale;
 };

@@ -149,26 +149,28 @@ __global__ void absmax_kernel(const double *__restrict__ x, long long n, unsigne
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-// x in [-1, 1] -> x0 + x1 + x2 with x_i = n_i 2^(-7-8i), |n_i| <= 128: exact in bf16, so every
-// slice product n_i m_j 2^(-14-8(i+j)) is exact in FP32; the remainder is below 2^-24.
+// x -> x0 + x1 + x2, each slice the bf16 rounding of what the previous ones left: every
+// slice is exact in bf16 (8 significant bits) and the three carry about 26 bits of x
+// relative to |x| itself, so small elements keep their precision next to large ones.  A
+// product of two slices has at most 16 significant bits and is exact in FP32; the dropped
+// products (x1 y2, x2 y1, x2 y2) and the remainder are below 2^-26 |x y|.
 __device__ __forceinline__ void split3(double x, __bf16 &a0, __bf16 &a1, __bf16 &a2)
 {
-    const double n0 = rint(x * 128.0);
-    double r = x - n0 * (1.0 / 128.0);
-    const double n1 = rint(r * 32768.0);
-    r -= n1 * (1.0 / 32768.0);
-    const double n2 = rint(r * 8388608.0);
-    a0 = (__bf16)(float)(n0 * (1.0 / 128.0));
-    a1 = (__bf16)(float)(n1 * (1.0 / 32768.0));
-    a2 = (__bf16)(float)(n2 * (1.0 / 8388608.0));
+    a0 = (__bf16)(float)x;
+    double r = x - (double)(float)a0;
+    a1 = (__bf16)(float)r;
+    r -= (double)(float)a1;
+    a2 = (__bf16)(float)r;
 }
 
-// Slice planes in v_mfma_f32_16x16x32_bf16 fragment order: panel, outer index (batch), K block
-// of 32, lane (inner index = lane & 15, K = 32 kb + 8 (lane >> 4) + e), e.  BX = 1: the source
-// is the BX slab [panel][K][16 gamma] and there is no outer index.
+// Split slabs in v_mfma_f32_16x16x32_bf16 fragment order, the three slices of one K block
+// side by side: AX [panel][outer][K block of 32][slice][lane][e], BX [panel][K block][slice]
+// [lane][e], with inner index = lane & 15 and K = 32 kb + 8 (lane >> 4) + e.  One panel is one
+// contiguous range, so a wave reaches all its fragments from one uniform base pointer.  BX = 1:
+// the source is the BX slab [panel][K][16 gamma] and there is no outer index.
 template <int BX>
 __global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__restrict__ dst, long long npanel,
-                                  int kp, int kps, double inv_scale, long long slice)
+                                  int kp, int kps, double inv_scale)
 {
     const int nkb = kps / 32;
     const long long per_panel = (BX ? 16LL : 256LL) * kps;
@@ -176,26 +178,27 @@ __global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__rest
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
         const int e = (int)(idx & 7), lane = (int)((idx >> 3) & 63);
-        long long rr = idx >> 9;
+        long long rr = idx >> 9;           // (panel, outer, kb) or (panel, kb)
         const int kb = (int)(rr % nkb);
-        rr /= nkb;
+        const long long po = rr / nkb;     // panel * 16 + outer, or panel
         const int K = kb * 32 + (lane >> 4) * 8 + e, inner = lane & 15;
         double v = 0.0;
         if (K < kp) {
             if (BX) {
-                v = src[rr * (16LL * kp) + (long long)K * 16 + inner];
+                v = src[po * (16LL * kp) + (long long)K * 16 + inner];
             } else {
-                const int outer = (int)(rr % 16);
-                const long long panel = rr / 16;
+                const int outer = (int)(po % 16);
+                const long long panel = po / 16;
                 v = src[panel * (256LL * kp) + (((long long)(outer >> 1) * (kp / 4) + K / 4) * 64 + (K & 3) * 16 + inner) * 2 +
                         (outer & 1)];
             }
         }
         __bf16 a0, a1, a2;
         split3(v * inv_scale, a0, a1, a2);
-        dst[idx] = a0;
-        dst[idx + slice] = a1;
-        dst[idx + 2 * slice] = a2;
+        const long long o = (rr * 3) * 512 + lane * 8 + e;
+        dst[o] = a0;
+        dst[o + 512] = a1;
+        dst[o + 1024] = a2;
     }
 }
 
@@ -216,15 +219,37 @@ __host__ __device__ constexpr int nth_bit(int mask, int n)
     return -1;
 }
 
+// Terms of one phase that read AX slot s: ordering p and BX operand b of each.
+struct SlotTerms {
+    int n;
+    int p[12], b[12];
+};
+__host__ __device__ constexpr SlotTerms slot_terms(int phase, int mode, int s)
+{
+    SlotTerms r{0, {}, {}};
+    const int mm = mode_mask(mode);
+    for (int p = 0; p < 6; ++p)
+        if ((mm >> p) & 1) {
+            const Term t = term(phase, p);
+            if (t.s1 == s) { r.p[r.n] = p; r.b[r.n] = t.b1; ++r.n; }
+            if (t.s2 == s) { r.p[r.n] = p; r.b[r.n] = t.b2; ++r.n; }
+        }
+    return r;
+}
+
+typedef const __attribute__((address_space(1))) bf16x8 gbf8;   // global memory, 16 B per lane
+
 // Blocks of 32 K: the BX slices of the phase (shared by all waves) are staged in LDS, double
 // buffered across blocks; the AX slices stream per operand slot through a two-deep register
 // ring (two blocks per loop iteration keep the ring index a compile-time constant).  Each
-// slot's MFMAs are the 6 slice products i + j <= 2 of its terms, FP32 accumulation.
+// slot step reads the BX fragments of its terms from LDS first, then issues the 6 slice
+// products i + j <= 2 of every term and batch, smallest first, with consecutive MFMAs on
+// different accumulators.  Loads are global-address-space with one uniform base per operand,
+// so they count only against the vector-memory counter and never delay an LDS wait.
 template <int PHASE, int MODE>
 __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const ItemCtx &it, f4 (&acc)[6][BPW],
                                                 __bf16 *lbx, int wave, int lane)
 {
-    constexpr int MM = mode_mask(MODE);
     constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
     constexpr int NSL = __builtin_popcount(SM), NBX = __builtin_popcount(XM);
     const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
@@ -234,29 +259,30 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
     else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
     else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
     const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
-    const long long kps = g.kps;
-    const __bf16 *ap[6];
+    const int nkb = g.kps / 32;
+    // units of 16 B: an AX panel is 16 outer rows x nkb blocks x 3 slices x 64 lanes = 96 kps,
+    // a BX panel nkb x 3 x 64 = 6 kps; this wave's outer rows are 2 wave and 2 wave + 1
+    gbf8 *ap[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int o = s / 3, xp = s % 3;
-        ap[s] = static_cast<const __bf16 *>(col[o] ? g.axc_s : g.axr_s) +
-                (((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o]) * (256LL * kps) + (long long)wave * (32LL * kps);
+        const long long panel = ((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o];
+        ap[s] = (gbf8 *)(col[o] ? g.axc_s : g.axr_s) + panel * (96LL * g.kps) +
+                (long long)wave * (2 * nkb * 192);
     }
-    const __bf16 *xq[6];
+    gbf8 *xq[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q)
-        xq[q] = static_cast<const __bf16 *>(g.bx_s) + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (16LL * kps);
-    const int nkb = g.kps / 32;
+        xq[q] = (gbf8 *)g.bx_s + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (6LL * g.kps);
     const int lo = lane * 8;
 
-    // BX staging: chunk c = rank*3 + slice (64 lanes x 16 B) is loaded by wave c % NWAVE
+    // BX staging: chunk c = rank * 3 + slice (64 lanes x 16 B) is loaded by wave c % NWAVE
     bf16x8 br[3];
     auto bx_load = [&](int kb) __attribute__((always_inline)) {
         static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
             constexpr int c = decltype(Cc)::value;
             constexpr int q = nth_bit(XM, c / 3), sl = c % 3;
-            if (wave == c % NWAVE)
-                br[c / NWAVE] = *reinterpret_cast<const bf16x8 *>(xq[q] + sl * g.bx_slice + (long long)kb * 512 + lo);
+            if (wave == c % NWAVE) br[c / NWAVE] = xq[q][(kb * 3 + sl) * 64 + lane];
         });
     };
     auto bx_store = [&](int buf) __attribute__((always_inline)) {
@@ -270,43 +296,35 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
 #pragma unroll
         for (int bb = 0; bb < BPW; ++bb)
 #pragma unroll
-            for (int sl = 0; sl < 3; ++sl)
-                d[bb][sl] = *reinterpret_cast<const bf16x8 *>(ap[s] + (long long)bb * (16LL * kps) + sl * g.ax_slice +
-                                                              (long long)kb * 512 + lo);
+            for (int sl = 0; sl < 3; ++sl) d[bb][sl] = ap[s][((bb * nkb + kb) * 3 + sl) * 64 + lane];
     };
     auto compute = [&](auto Sc, const bf16x8 (&a)[BPW][3], int buf) __attribute__((always_inline)) {
         constexpr int s = decltype(Sc)::value;
-        static_for<6>([&](auto P) __attribute__((always_inline)) {
-            constexpr int p = decltype(P)::value;
-            if constexpr ((MM >> p) & 1) {
-                constexpr Term t = term(PHASE, p);
-                static_for<2>([&](auto U) __attribute__((always_inline)) {
-                    constexpr int u = decltype(U)::value;
-                    constexpr int ts = u == 0 ? t.s1 : t.s2, tb = u == 0 ? t.b1 : t.b2;
-                    if constexpr (ts == s) {
-                        constexpr int rank = __builtin_popcount(XM & ((1 << tb) - 1));
-                        bf16x8 bs[3];
+        constexpr SlotTerms ST = slot_terms(PHASE, MODE, s);
+        bf16x8 bs[ST.n > 0 ? ST.n : 1][3];
+        static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+            constexpr int n = decltype(N)::value;
+            constexpr int rank = __builtin_popcount(XM & ((1 << ST.b[n]) - 1));
 #pragma unroll
-                        for (int sl = 0; sl < 3; ++sl)
-                            bs[sl] = *reinterpret_cast<const bf16x8 *>(lbx + ((buf * NBX + rank) * 3 + sl) * 512 + lo);
+            for (int sl = 0; sl < 3; ++sl)
+                bs[n][sl] = *reinterpret_cast<const bf16x8 *>(lbx + ((buf * NBX + rank) * 3 + sl) * 512 + lo);
+        });
+        // slice products with i + j <= 2, smallest first
+        static_for<6>([&](auto Q) __attribute__((always_inline)) {
+            constexpr int qq = 5 - decltype(Q)::value;
+            constexpr int si = qq == 0 ? 0 : qq == 1 ? 0 : qq == 2 ? 1 : qq == 3 ? 0 : qq == 4 ? 1 : 2;
+            constexpr int sj = qq == 0 ? 0 : qq == 1 ? 1 : qq == 2 ? 0 : qq == 3 ? 2 : qq == 4 ? 1 : 0;
+            static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+                constexpr int n = decltype(N)::value;
+                constexpr int p = ST.p[n];
 #pragma unroll
-                        for (int bb = 0; bb < BPW; ++bb) {
-                            f4 c = acc[p][bb];
-                            // slice products with i + j <= 2, smallest first
-                            static_for<6>([&](auto Q) __attribute__((always_inline)) {
-                                constexpr int qq = 5 - decltype(Q)::value;
-                                constexpr int si = qq == 0 ? 0 : qq == 1 ? 0 : qq == 2 ? 1 : qq == 3 ? 0 : qq == 4 ? 1 : 2;
-                                constexpr int sj = qq == 0 ? 0 : qq == 1 ? 1 : qq == 2 ? 0 : qq == 3 ? 2 : qq == 4 ? 1 : 0;
-                                if constexpr (PHASE == 2)
-                                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bs[sj], a[bb][si], c, 0, 0, 0);
-                                else
-                                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[bb][si], bs[sj], c, 0, 0, 0);
-                            });
-                            acc[p][bb] = c;
-                        }
-                    }
-                });
-            }
+                for (int bb = 0; bb < BPW; ++bb) {
+                    if constexpr (PHASE == 2)
+                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bs[n][sj], a[bb][si], acc[p][bb], 0, 0, 0);
+                    else
+                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[bb][si], bs[n][sj], acc[p][bb], 0, 0, 0);
+                }
+            });
         });
     };
 
@@ -751,8 +769,7 @@ int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *v
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
-                    int ntup, int nblk, int grid, const void *const *split, long long ax_slice, long long bx_slice,
-                    int kps, double wscale, void *stream)
+                    int ntup, int nblk, int grid, const void *const *split, int kps, double wscale, void *stream)
 {
     static bool attr_set[3] = {false, false, false};
     const size_t lds_bytes = sizeof(double) * LDS_DOUBLES;
@@ -773,7 +790,7 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     g.axr_s = form == 2 ? split[0] : nullptr;
     g.axc_s = form == 2 ? split[1] : nullptr;
     g.bx_s = form == 2 ? split[2] : nullptr;
-    g.ax_slice = ax_slice; g.bx_slice = bx_slice; g.kps = kps;
+    g.kps = kps;
     g.wscale = form == 2 ? wscale : 1.0;
     {
         const char *env = getenv("CCX_ITEM_ORDER");
@@ -790,8 +807,9 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// Split BF16 slabs from the FP64 ones: power-of-two scales sA >= max|AX|, sB >= max|BX|, three
-// fixed-point bf16 slices of AX/sA (both orientations) and BX/sB; *wscale = sA * sB.
+// Split BF16 slabs from the FP64 ones: power-of-two scales sA >= max|AX|, sB >= max|BX| (they
+// keep FP64 inputs of any range inside FP32's), three bf16 slices of AX/sA (both orientations)
+// and BX/sB; *wscale = sA * sB.
 int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, void *axr_s, void *axc_s, void *bx_s,
                        unsigned long long *dmax, int no, int nb, int kp, int kps, double *wscale, void *stream)
 {
@@ -811,13 +829,10 @@ int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, v
         if (m > 0.0) frexp(m, &e);   // m < 2^e
         sc[q] = m > 0.0 ? ldexp(1.0, e) : 1.0;
     }
-    const long long ax_slice = (long long)no * nb * nb * 256 * kps, bx_slice = (long long)no * no * nb * 16 * kps;
-    pack_split_kernel<0><<<grid_for(ax_slice), 256, 0, s>>>(axr, (__bf16 *)axr_s, (long long)no * nb * nb, kp, kps,
-                                                             1.0 / sc[0], ax_slice);
-    pack_split_kernel<0><<<grid_for(ax_slice), 256, 0, s>>>(axc, (__bf16 *)axc_s, (long long)no * nb * nb, kp, kps,
-                                                             1.0 / sc[0], ax_slice);
-    pack_split_kernel<1><<<grid_for(bx_slice), 256, 0, s>>>(bx, (__bf16 *)bx_s, (long long)no * no * nb, kp, kps,
-                                                             1.0 / sc[1], bx_slice);
+    const long long ax_n = (long long)no * nb * nb * 256 * kps, bx_n = (long long)no * no * nb * 16 * kps;
+    pack_split_kernel<0><<<grid_for(ax_n), 256, 0, s>>>(axr, (__bf16 *)axr_s, (long long)no * nb * nb, kp, kps, 1.0 / sc[0]);
+    pack_split_kernel<0><<<grid_for(ax_n), 256, 0, s>>>(axc, (__bf16 *)axc_s, (long long)no * nb * nb, kp, kps, 1.0 / sc[0]);
+    pack_split_kernel<1><<<grid_for(bx_n), 256, 0, s>>>(bx, (__bf16 *)bx_s, (long long)no * no * nb, kp, kps, 1.0 / sc[1]);
     *wscale = sc[0] * sc[1];
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }

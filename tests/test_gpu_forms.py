@@ -1,9 +1,8 @@
 """The reduced-precision arithmetic forms against the FP64 form of the same library.
 
-FP32 (FP32 operands, FP32 MFMA accumulation) and the split BF16 form (three fixed-point
-bf16 slices per FP64 operand under one power-of-two scale per operand slab, six slice
-products accumulated in FP32, W rescaled in FP64) both keep the denominators, the epilogue
-and every reduction in FP64.  Their error against FP64 is that of FP32 accumulation:
+FP32 (FP32 operands, FP32 MFMA accumulation) and the split BF16 form (hi + mid + lo bf16
+slices per FP64 operand element, six slice products accumulated in FP32, W rescaled in
+FP64) both keep the denominators, the epilogue and every reduction in FP64.  Their error against FP64 is that of FP32 accumulation:
 the bar here is 1e-6 relative on E[T] and E_ST and 1e-5 of max|v1| on v1 (v1 sums W
 against T1-sized factors with cancellation, so it keeps fewer correct digits)."""
 

@@ -38,6 +38,7 @@ extern "C" {
 #define CCX_ERR_STATE 4    /* call out of order (no inputs loaded, no init)   */
 #define CCX_ERR_INPUT 5    /* NaN/Inf in an input array                       */
 #define CCX_ERR_FORM 6     /* arithmetic form not available                   */
+#define CCX_ERR_IO 7       /* restart file missing, short or mis-framed       */
 
 /* arithmetic forms of the triples contraction; denominators, energies, v1 and all
  * reductions are FP64 in every form */
@@ -180,6 +181,65 @@ void trant3_5_(int64_t *n, double *v);
 void trant3_acc_(double *v, int64_t *nu, int64_t *id);
 void trant3_smp_(double *v, int64_t *nu, int64_t *id);
 void tranmd_23_(double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4);
+
+/* trant3_1_async / trant3_4_async(acc_sync,n,v): src/triples-ijk/ijk_tuple_acc.f90:109-153, and
+ * tranmd_23_acc(acc_sync,a,n1,n2,n3,n4): src/common/common_cc.f90:203-237.  The same
+ * permutations as trant3_1_ / trant3_4_ / tranmd_23_ on host memory; the OpenACC queue id is
+ * not used (the call completes before it returns). */
+void trant3_1_async_(int64_t *acc_sync, int64_t *n, double *v);
+void trant3_4_async_(int64_t *acc_sync, int64_t *n, double *v);
+void tranmd_23_acc_(int64_t *acc_sync, double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4);
+
+/* TRANMD_SMP(a,n1,n2,n3,n4,ij): src/triples-ijk/common_functions.f:141-421.  In-place
+ * permutation of a(n1,n2,n3,n4), new(x1,x2,x3,x4) = old(source) by code ij:
+ *   12 (x2,x1,x3,x4)  13 (x3,x2,x1,x4)  14 (x4,x2,x3,x1)  23 (x1,x3,x2,x4)  24 (x1,x4,x3,x2)
+ *   34 (x1,x2,x4,x3)  231 (x2,x3,x1,x4)  312 (x3,x1,x2,x4)  341 (x3,x2,x4,x1)
+ *   413 (x4,x2,x1,x3)  1234 (x4,x3,x2,x1)
+ * The permuted extents must be equal (CCX_ERR_ARG otherwise, array unchanged); any other
+ * code leaves the array as it is, like the reference. */
+void tranmd_smp_(double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4, int64_t *ij);
+
+/* DDCC_T_IJK_BIG(no,nu,i,j,k,t1,t2,vm,v3,t3,voe,o1,tmp,eh,ep,ve_i,ve_j,ve_k): the triples-abc
+ * per-tuple routine, src/triples-abc/ddcc_t_ijk_big.f:1-32 (driver call
+ * src/triples-abc/triples.f90:255 passes t1 = the v1 accumulator, o1 = T1).  Adds the distinct
+ * tuple's E[T] to etd and its v1 rows to t1, the same sums as ddcc_t_ijk_acc_; v3 receives W;
+ * t3, o1 and tmp are not read. */
+void ddcc_t_ijk_big_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, int64_t *k, double *t1, double *t2,
+                     double *vm, double *v3, double *t3, double *voe, double *o1, double *tmp, double *eh, double *ep,
+                     double *ve_i, double *ve_j, double *ve_k);
+
+/* The reference's CUDA tuple path: ijk_gpu_init(no,nu,eh,ep,v1,t2,vm,voe),
+ * ijk_gpu_driver(nu,no,i,j,k,ve_i,ve_j,ve_k,t1,eh,ep,etd), ijk_gpu_finalize(no,nu,etd)
+ * (call sites src/triples-ijk/triples_cpu.F90:363,497 and ddcc_t_ijk_big_cuda.f:12-14).  init
+ * keeps the caller's arrays and clears the accumulators; driver takes each occupied index's
+ * VVVO patch as the distributed array holds it (nutr x nu: column c is the lower-packed (p,q)
+ * block of VVVO(:,:,c,x)) and adds the distinct tuple's E[T] and v1 rows to the accumulators;
+ * finalize adds them to *etd and to the v1 given to init.  ccx_last_status() reports errors. */
+void ijk_gpu_init_(int64_t *no, int64_t *nu, double *eh, double *ep, double *v1, double *t2, double *vm,
+                   double *voe);
+void ijk_gpu_driver_(int64_t *nu, int64_t *no, int64_t *i, int64_t *j, int64_t *k, double *ve_i, double *ve_j,
+                     double *ve_k, double *t1, double *eh, double *ep, double *etd);
+void ijk_gpu_finalize_(int64_t *no, int64_t *nu, double *etd);
+
+/* ---------------------------------------------------------------- triples.restart */
+
+/* The reference's input file (src/triples-ijk/program.F90:44-46 reads no and nu,
+ * cc_triples_readinp :195-218 the arrays): Fortran unformatted sequential records, each
+ * framed by a 4-byte byte count before and after, integers int64.  Records: no; nu; eh(no);
+ * ep(nu); t1(no*nu); t2(no^2 nu^2); vm(no^3 nu); voe(no^2 nu^2); then no*nu records of
+ * nutr = nu(nu+1)/2 doubles, record j = column j = c + nu*x of the lower-packed VVVO (rows
+ * p(p+1)/2 + q, q <= p, 0-based).  All arrays in the reference layouts.  Returns CCX_ERR_IO
+ * for a missing, short or mis-framed file. */
+int ccx_restart_dims(const char *path, int64_t *no, int64_t *nu);
+/* Reads every array; vvvo_packed receives (nutr, no*nu). */
+int ccx_read_restart(const char *path, int64_t no, int64_t nu, double *eh, double *ep, double *t1, double *t2,
+                     double *vm, double *voe, double *vvvo_packed);
+/* Writes the file; vvvo in either layout of ccx_load_host (0 = (nu,nu,nu,no), 1 = packed). */
+int ccx_write_restart(const char *path, int64_t no, int64_t nu, const double *eh, const double *ep,
+                      const double *t1, const double *t2, const double *vm, const double *voe, const double *vvvo,
+                      int vvvo_layout);
+/* ccx_load_host from a restart file whose no, nu match the context's. */
+int ccx_load_restart(ccx_ctx *ctx, const char *path);
 
 #ifdef __cplusplus
 }

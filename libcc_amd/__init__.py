@@ -94,5 +94,29 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).restype = None
     L.tranmd_23_.argtypes = [dp, pi64, pi64, pi64, pi64]
     L.tranmd_23_.restype = None
+    for name in ("trant3_1_async_", "trant3_4_async_"):
+        getattr(L, name).argtypes = [pi64, pi64, dp]
+        getattr(L, name).restype = None
+    L.tranmd_23_acc_.argtypes = [pi64, dp, pi64, pi64, pi64, pi64]
+    L.tranmd_23_acc_.restype = None
+    L.tranmd_smp_.argtypes = [dp, pi64, pi64, pi64, pi64, pi64]
+    L.tranmd_smp_.restype = None
+    L.ddcc_t_ijk_big_.argtypes = [pi64] * 5 + [dp] * 13
+    L.ddcc_t_ijk_big_.restype = None
+    L.ijk_gpu_init_.argtypes = [pi64, pi64] + [dp] * 6
+    L.ijk_gpu_init_.restype = None
+    L.ijk_gpu_driver_.argtypes = [pi64] * 5 + [dp] * 7
+    L.ijk_gpu_driver_.restype = None
+    L.ijk_gpu_finalize_.argtypes = [pi64, pi64, dp]
+    L.ijk_gpu_finalize_.restype = None
+    cp = ctypes.c_char_p
+    L.ccx_restart_dims.argtypes = [cp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.ccx_restart_dims.restype = ctypes.c_int
+    L.ccx_read_restart.argtypes = [cp, i64, i64] + [dp] * 7
+    L.ccx_read_restart.restype = ctypes.c_int
+    L.ccx_write_restart.argtypes = [cp, i64, i64] + [dp] * 7 + [ctypes.c_int]
+    L.ccx_write_restart.restype = ctypes.c_int
+    L.ccx_load_restart.argtypes = [ctxp, cp]
+    L.ccx_load_restart.restype = ctypes.c_int
     _LIB = L
     return L

@@ -4,6 +4,7 @@
  * reference's triples path.  All device work goes through the thin ABI in ccx_dev.h.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -44,6 +45,7 @@ const char *ccx_error_string(int s)
     case CCX_ERR_STATE: return "call out of order";
     case CCX_ERR_INPUT: return "non-finite value in an input array";
     case CCX_ERR_FORM: return "arithmetic form not available";
+    case CCX_ERR_IO: return "file missing, unreadable or not in the restart layout";
     default: return "unknown status";
     }
 }
@@ -961,4 +963,280 @@ void tranmd_23_(double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4)
                     *q = x;
                 }
     g_status = CCX_OK;
+}
+
+/* trant3_1_async(acc_sync,n,v), trant3_4_async(acc_sync,n,v): src/triples-ijk/ijk_tuple_acc.f90:109-153.
+ * The queue id selects an OpenACC queue in the reference; these host permutations finish
+ * before returning, so it is not read. */
+void trant3_1_async_(int64_t *q, int64_t *n, double *v)
+{
+    (void)q;
+    g_status = n ? cube_perm(v, *n, 1) : CCX_ERR_ARG;
+}
+void trant3_4_async_(int64_t *q, int64_t *n, double *v)
+{
+    (void)q;
+    g_status = n ? cube_perm(v, *n, 4) : CCX_ERR_ARG;
+}
+
+/* tranmd_23_acc(acc_sync,a,n1,n2,n3,n4): src/common/common_cc.f90:203-237, tranmd_23 on a queue */
+void tranmd_23_acc_(int64_t *q, double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4)
+{
+    (void)q;
+    tranmd_23_(a, n1, n2, n3, n4);
+}
+
+/* TRANMD_SMP(a,n1,n2,n3,n4,ij): src/triples-ijk/common_functions.f:141-421.  In-place index
+ * permutation of a(n1,n2,n3,n4), new(x1,x2,x3,x4) = old(source), by code:
+ *   12 (x2,x1,x3,x4)  13 (x3,x2,x1,x4)  14 (x4,x2,x3,x1)  23 (x1,x3,x2,x4)  24 (x1,x4,x3,x2)
+ *   34 (x1,x2,x4,x3)  231 (x2,x3,x1,x4)  312 (x3,x1,x2,x4)  341 (x3,x2,x4,x1)  413 (x4,x2,x1,x3)
+ *   1234 (x4,x3,x2,x1)
+ * Each needs the permuted extents equal (the reference's loops assume it); other extents
+ * are rejected with CCX_ERR_ARG and the array left as it was.  Any other code is a no-op,
+ * as in the reference. */
+void tranmd_smp_(double *a, int64_t *n1, int64_t *n2, int64_t *n3, int64_t *n4, int64_t *ij)
+{
+    if (!a || !n1 || !n2 || !n3 || !n4 || !ij || *n1 < 0 || *n2 < 0 || *n3 < 0 || *n4 < 0) {
+        g_status = CCX_ERR_ARG;
+        return;
+    }
+    static const struct { int code; int src[4]; } tab[] = {
+        {12, {1, 0, 2, 3}},  {13, {2, 1, 0, 3}},  {14, {3, 1, 2, 0}},  {23, {0, 2, 1, 3}},
+        {24, {0, 3, 2, 1}},  {34, {0, 1, 3, 2}},  {231, {1, 2, 0, 3}}, {312, {2, 0, 1, 3}},
+        {341, {2, 1, 3, 0}}, {413, {3, 1, 0, 2}}, {1234, {3, 2, 1, 0}},
+    };
+    int e = -1;
+    for (int q = 0; q < (int)(sizeof(tab) / sizeof(tab[0])); ++q)
+        if (tab[q].code == *ij) e = q;
+    if (e < 0) { g_status = CCX_OK; return; }
+    const int64_t n[4] = {*n1, *n2, *n3, *n4};
+    const int *s = tab[e].src;
+    for (int d = 0; d < 4; ++d)
+        if (n[s[d]] != n[d]) { g_status = CCX_ERR_ARG; return; }
+    const size_t total = (size_t)(n[0] * n[1] * n[2] * n[3]);
+    double *old = (double *)malloc(sizeof(double) * (total ? total : 1));
+    if (!old) { g_status = CCX_ERR_ALLOC; return; }
+    memcpy(old, a, sizeof(double) * total);
+    const int64_t str[4] = {1, n[0], n[0] * n[1], n[0] * n[1] * n[2]};
+    int64_t x[4];
+    for (x[3] = 0; x[3] < n[3]; ++x[3])
+        for (x[2] = 0; x[2] < n[2]; ++x[2])
+            for (x[1] = 0; x[1] < n[1]; ++x[1])
+                for (x[0] = 0; x[0] < n[0]; ++x[0]) {
+                    const int64_t dst = x[0] + str[1] * x[1] + str[2] * x[2] + str[3] * x[3];
+                    const int64_t src = x[s[0]] + str[1] * x[s[1]] + str[2] * x[s[2]] + str[3] * x[s[3]];
+                    a[dst] = old[src];
+                }
+    free(old);
+    g_status = CCX_OK;
+}
+
+/* DDCC_T_IJK_BIG(no,nu,i,j,k,t1,t2,vm,v3,t3,voe,o1,tmp,eh,ep,ve_i,ve_j,ve_k), the triples-abc
+ * per-tuple routine (src/triples-abc/ddcc_t_ijk_big.f:1-32, called with t1 = v1 and o1 = T1 at
+ * src/triples-abc/triples.f90:255).  Its T3SQUA_SMP energy and DRT1WT3IJK_SMP v1 update are
+ * the same sums as t1wt3_ijk's (weight 1 for i > j > k), so it shares ddcc_t_ijk_acc_'s
+ * implementation; v3 receives W (the reference leaves W/D, permuted), t3, o1 and tmp are
+ * not read. */
+void ddcc_t_ijk_big_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, int64_t *k, double *t1, double *t2,
+                     double *vm, double *v3, double *t3, double *voe, double *o1, double *tmp, double *eh, double *ep,
+                     double *ve_i, double *ve_j, double *ve_k)
+{
+    (void)t3; (void)o1; (void)tmp;
+    ddcc_t_ijk_acc_(no, nu, i, j, k, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_j, ve_k);
+}
+
+/* ------------------------------------------------------------------ the CUDA driver trio */
+
+/* ijk_gpu_init(no,nu,eh,ep,v1,t2,vm,voe) / ijk_gpu_driver(nu,no,i,j,k,ve_i,ve_j,ve_k,t1,eh,ep,etd)
+ * / ijk_gpu_finalize(no,nu,etd): the reference's CUDA tuple path (call sites
+ * src/triples-ijk/triples_cpu.F90:363,465,497 and ddcc_t_ijk_big_cuda.f:1-16; its sources are
+ * not in the tree).  init keeps the caller's arrays and clears the accumulators; driver takes
+ * each occupied index's raw VVVO patch as the distributed array holds it (nutr x nu, column c
+ * = the lower-packed (p,q) block of VVVO(:,:,c,x)) and does the expansion and index swap
+ * itself, then adds the tuple's E[T] and v1 contributions to the accumulators; finalize adds
+ * them to *etd and to the v1 passed to init.  The driver's own t1/etd arguments are not read. */
+static struct {
+    int active;
+    int64_t no, nu;
+    double *v1;
+    const double *eh, *ep, *t2, *vm, *voe;
+    double *acc_v1, *ve[3], *v3;
+    double etd;
+} g_gpu;
+
+static void gpu_trio_free(void)
+{
+    free(g_gpu.acc_v1);
+    for (int q = 0; q < 3; ++q) free(g_gpu.ve[q]);
+    free(g_gpu.v3);
+    memset(&g_gpu, 0, sizeof(g_gpu));
+}
+
+void ijk_gpu_init_(int64_t *no, int64_t *nu, double *eh, double *ep, double *v1, double *t2, double *vm, double *voe)
+{
+    if (!no || !nu || *no < 1 || *nu < 1 || !eh || !ep || !v1 || !t2 || !vm || !voe) { g_status = CCX_ERR_ARG; return; }
+    gpu_trio_free();
+    const size_t nu3 = (size_t)(*nu * *nu * *nu);
+    g_gpu.acc_v1 = (double *)calloc((size_t)(*no * *nu), sizeof(double));
+    for (int q = 0; q < 3; ++q) g_gpu.ve[q] = (double *)malloc(sizeof(double) * nu3);
+    g_gpu.v3 = (double *)malloc(sizeof(double) * nu3);
+    if (!g_gpu.acc_v1 || !g_gpu.ve[0] || !g_gpu.ve[1] || !g_gpu.ve[2] || !g_gpu.v3) {
+        gpu_trio_free();
+        g_status = CCX_ERR_ALLOC;
+        return;
+    }
+    g_gpu.no = *no; g_gpu.nu = *nu;
+    g_gpu.v1 = v1; g_gpu.eh = eh; g_gpu.ep = ep; g_gpu.t2 = t2; g_gpu.vm = vm; g_gpu.voe = voe;
+    g_gpu.etd = 0.0;
+    g_gpu.active = 1;
+    g_status = CCX_OK;
+}
+
+void ijk_gpu_driver_(int64_t *nu, int64_t *no, int64_t *i, int64_t *j, int64_t *k, double *ve_i, double *ve_j,
+                     double *ve_k, double *t1, double *eh, double *ep, double *etd)
+{
+    (void)t1; (void)eh; (void)ep; (void)etd;
+    if (!g_gpu.active) { g_status = CCX_ERR_STATE; return; }
+    if (!nu || !no || !i || !j || !k || !ve_i || !ve_j || !ve_k || *nu != g_gpu.nu || *no != g_gpu.no) {
+        g_status = CCX_ERR_ARG;
+        return;
+    }
+    const int64_t n = g_gpu.nu, nutr = n * (n + 1) / 2;
+    const double *patch[3] = {ve_i, ve_j, ve_k};
+    for (int s = 0; s < 3; ++s)   /* VE_x(d,p,q) = VVVO(d,q,p,x) = patch(tri(d,q), p) */
+        for (int64_t q = 0; q < n; ++q)
+            for (int64_t p = 0; p < n; ++p)
+                for (int64_t d = 0; d < n; ++d) {
+                    const int64_t hi = d > q ? d : q, lo = d > q ? q : d;
+                    g_gpu.ve[s][d + n * (p + n * q)] = patch[s][hi * (hi + 1) / 2 + lo + nutr * p];
+                }
+    const double etd0 = g_etd;
+    g_status = tuple_step(g_gpu.no, n, *i - 1, *j - 1, *k - 1, g_gpu.acc_v1, g_gpu.t2, g_gpu.vm, g_gpu.v3,
+                          g_gpu.voe, g_gpu.eh, g_gpu.ep, g_gpu.ve[0], g_gpu.ve[1], g_gpu.ve[2], 1.0);
+    g_gpu.etd += g_etd - etd0;
+    g_etd = etd0;
+}
+
+void ijk_gpu_finalize_(int64_t *no, int64_t *nu, double *etd)
+{
+    if (!g_gpu.active) { g_status = CCX_ERR_STATE; return; }
+    if (!no || !nu || !etd || *no != g_gpu.no || *nu != g_gpu.nu) { g_status = CCX_ERR_ARG; return; }
+    *etd += g_gpu.etd;
+    for (int64_t q = 0; q < g_gpu.no * g_gpu.nu; ++q) g_gpu.v1[q] += g_gpu.acc_v1[q];
+    gpu_trio_free();
+    g_status = CCX_OK;
+}
+
+/* ------------------------------------------------------------------ triples.restart */
+
+/* The reference's input file (src/triples-ijk/program.F90:44-46,195-218): Fortran unformatted
+ * sequential records, each framed by a 4-byte length before and after, int64 integers:
+ * no; nu; eh(no); ep(nu); t1(no*nu); t2(no^2 nu^2); vm(no^3 nu); voe(no^2 nu^2); then no*nu
+ * records of nutr doubles, record j = column j = c + nu*x of the lower-packed VVVO.
+ * Records above 2^31-1 bytes (not representable with 4-byte markers) are rejected. */
+static int rec_read(FILE *f, void *buf, size_t bytes)
+{
+    int32_t m0 = 0, m1 = 0;
+    if (bytes > (size_t)INT32_MAX) return CCX_ERR_IO;
+    if (fread(&m0, 4, 1, f) != 1 || (size_t)m0 != bytes) return CCX_ERR_IO;
+    if (bytes && fread(buf, 1, bytes, f) != bytes) return CCX_ERR_IO;
+    if (fread(&m1, 4, 1, f) != 1 || m1 != m0) return CCX_ERR_IO;
+    return CCX_OK;
+}
+
+static int rec_write(FILE *f, const void *buf, size_t bytes)
+{
+    if (bytes > (size_t)INT32_MAX) return CCX_ERR_IO;
+    const int32_t m = (int32_t)bytes;
+    if (fwrite(&m, 4, 1, f) != 1 || (bytes && fwrite(buf, 1, bytes, f) != bytes) || fwrite(&m, 4, 1, f) != 1)
+        return CCX_ERR_IO;
+    return CCX_OK;
+}
+
+int ccx_restart_dims(const char *path, int64_t *no, int64_t *nu)
+{
+    if (!path || !no || !nu) return CCX_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return CCX_ERR_IO;
+    int rc = rec_read(f, no, 8);
+    if (rc == CCX_OK) rc = rec_read(f, nu, 8);
+    fclose(f);
+    if (rc == CCX_OK && (*no < 1 || *nu < 1)) rc = CCX_ERR_IO;
+    return rc;
+}
+
+int ccx_read_restart(const char *path, int64_t no, int64_t nu, double *eh, double *ep, double *t1, double *t2,
+                     double *vm, double *voe, double *vvvo_packed)
+{
+    if (!path || no < 1 || nu < 1 || !eh || !ep || !t1 || !t2 || !vm || !voe || !vvvo_packed) return CCX_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return CCX_ERR_IO;
+    int64_t fno = 0, fnu = 0;
+    const size_t nou = (size_t)(no * nu), no2u2 = (size_t)(no * no * nu * nu), nutr = (size_t)(nu * (nu + 1) / 2);
+    int rc = rec_read(f, &fno, 8);
+    if (rc == CCX_OK) rc = rec_read(f, &fnu, 8);
+    if (rc == CCX_OK && (fno != no || fnu != nu)) rc = CCX_ERR_ARG;
+    if (rc == CCX_OK) rc = rec_read(f, eh, 8 * (size_t)no);
+    if (rc == CCX_OK) rc = rec_read(f, ep, 8 * (size_t)nu);
+    if (rc == CCX_OK) rc = rec_read(f, t1, 8 * nou);
+    if (rc == CCX_OK) rc = rec_read(f, t2, 8 * no2u2);
+    if (rc == CCX_OK) rc = rec_read(f, vm, 8 * (size_t)(no * no * no * nu));
+    if (rc == CCX_OK) rc = rec_read(f, voe, 8 * no2u2);
+    for (size_t jcol = 0; rc == CCX_OK && jcol < nou; ++jcol) rc = rec_read(f, vvvo_packed + nutr * jcol, 8 * nutr);
+    fclose(f);
+    return rc;
+}
+
+int ccx_write_restart(const char *path, int64_t no, int64_t nu, const double *eh, const double *ep, const double *t1,
+                      const double *t2, const double *vm, const double *voe, const double *vvvo, int vvvo_layout)
+{
+    if (!path || no < 1 || nu < 1 || !eh || !ep || !t1 || !t2 || !vm || !voe || !vvvo ||
+        (vvvo_layout != 0 && vvvo_layout != 1))
+        return CCX_ERR_ARG;
+    const size_t nou = (size_t)(no * nu), no2u2 = (size_t)(no * no * nu * nu), nutr = (size_t)(nu * (nu + 1) / 2);
+    double *col = (double *)malloc(sizeof(double) * nutr);
+    if (!col) return CCX_ERR_ALLOC;
+    FILE *f = fopen(path, "wb");
+    if (!f) { free(col); return CCX_ERR_IO; }
+    int rc = rec_write(f, &no, 8);
+    if (rc == CCX_OK) rc = rec_write(f, &nu, 8);
+    if (rc == CCX_OK) rc = rec_write(f, eh, 8 * (size_t)no);
+    if (rc == CCX_OK) rc = rec_write(f, ep, 8 * (size_t)nu);
+    if (rc == CCX_OK) rc = rec_write(f, t1, 8 * nou);
+    if (rc == CCX_OK) rc = rec_write(f, t2, 8 * no2u2);
+    if (rc == CCX_OK) rc = rec_write(f, vm, 8 * (size_t)(no * no * no * nu));
+    if (rc == CCX_OK) rc = rec_write(f, voe, 8 * no2u2);
+    for (size_t jcol = 0; rc == CCX_OK && jcol < nou; ++jcol) {
+        const double *src = vvvo_layout == 1 ? vvvo + nutr * jcol : NULL;
+        if (vvvo_layout == 0) {   /* lower-packed (p >= q) rows of VVVO(:,:,c,x), p outer */
+            const double *blk = vvvo + (size_t)(nu * nu) * jcol;
+            size_t r = 0;
+            for (int64_t p = 0; p < nu; ++p)
+                for (int64_t q = 0; q <= p; ++q) col[r++] = blk[p + nu * q];
+            src = col;
+        }
+        rc = rec_write(f, src, 8 * nutr);
+    }
+    if (fclose(f) != 0 && rc == CCX_OK) rc = CCX_ERR_IO;
+    free(col);
+    return rc;
+}
+
+int ccx_load_restart(ccx_ctx *c, const char *path)
+{
+    if (!c || !path) return CCX_ERR_ARG;
+    int64_t no = 0, nu = 0;
+    int rc = ccx_restart_dims(path, &no, &nu);
+    if (rc != CCX_OK) return rc;
+    if (no != c->no || nu != c->nu) return CCX_ERR_ARG;
+    const size_t nou = (size_t)(no * nu), no2u2 = (size_t)(no * no * nu * nu), nutr = (size_t)(nu * (nu + 1) / 2);
+    double *eh = (double *)malloc(8 * (size_t)no), *ep = (double *)malloc(8 * (size_t)nu);
+    double *t1 = (double *)malloc(8 * nou), *t2 = (double *)malloc(8 * no2u2);
+    double *vm = (double *)malloc(8 * (size_t)(no * no * no * nu)), *voe = (double *)malloc(8 * no2u2);
+    double *vv = (double *)malloc(8 * nutr * nou);
+    if (!eh || !ep || !t1 || !t2 || !vm || !voe || !vv) rc = CCX_ERR_ALLOC;
+    if (rc == CCX_OK) rc = ccx_read_restart(path, no, nu, eh, ep, t1, t2, vm, voe, vv);
+    if (rc == CCX_OK) rc = ccx_load_host(c, eh, ep, t1, t2, vm, voe, vv, 1);
+    free(eh); free(ep); free(t1); free(t2); free(vm); free(voe); free(vv);
+    return rc;
 }

@@ -6,6 +6,7 @@
 ``t1`` in place (the legacy C entry ``cc_triples_acc_`` does, for drop-in
 fidelity).  ``TriplesContext`` exposes the context API: host or device-resident
 inputs (numpy arrays or torch tensors on the GPU), FP64 energies, per-tuple W.
+``read_restart`` / ``write_restart`` handle the reference's ``triples.restart`` file.
 """
 
 from __future__ import annotations
@@ -104,6 +105,17 @@ class TriplesContext:
         out.update({"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value})
         return out
 
+    def load_restart(self, path: str) -> None:
+        """Load a reference ``triples.restart`` file whose no, nu match (ccx_load_restart)."""
+        _check(lib().ccx_load_restart(self._ctx, str(path).encode()), "ccx_load_restart")
+
+    @classmethod
+    def from_restart(cls, path: str) -> "TriplesContext":
+        no, nu = restart_dims(path)
+        ctx = cls(no, nu)
+        ctx.load_restart(path)
+        return ctx
+
     def tuple_w(self, i: int, j: int, k: int) -> np.ndarray:
         """W_ijk cube for a 1-based tuple (no W(a,a,a)=0 step)."""
         w = np.zeros((self.nu, self.nu, self.nu), order="F")
@@ -132,6 +144,37 @@ def cc_triples(eh, ep, t1, t2, vm, voe, vvvo, form: str = "fp64", packed_vvvo: b
     finally:
         ctx.close()
     return r["ets"], r["etd"], r["v1"]
+
+
+def restart_dims(path: str) -> tuple:
+    """(no, nu) from the header records of a ``triples.restart`` file (ccx_restart_dims)."""
+    no, nu = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().ccx_restart_dims(str(path).encode(), ctypes.byref(no), ctypes.byref(nu)), "ccx_restart_dims")
+    return no.value, nu.value
+
+
+def read_restart(path: str) -> dict:
+    """Every array of a ``triples.restart`` file in the reference layouts (Fortran-ordered
+    numpy arrays); ``vvvo`` is the packed (nutr, no*nu) array the file holds."""
+    no, nu = restart_dims(path)
+    nutr = nu * (nu + 1) // 2
+    shapes = {"eh": (no,), "ep": (nu,), "t1": (no, nu), "t2": (nu, nu, no, no), "vm": (no, nu, no, no),
+              "voe": (nu, nu, no, no), "vvvo": (nutr, no * nu)}
+    out = {k: np.zeros(v, order="F") for k, v in shapes.items()}
+    _check(lib().ccx_read_restart(str(path).encode(), no, nu,
+                                  *[out[k].ctypes.data_as(_dp) for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")]),
+           "ccx_read_restart")
+    out["no"], out["nu"] = no, nu
+    return out
+
+
+def write_restart(path: str, inp: dict, packed_vvvo: bool = False) -> None:
+    """Write the arrays (reference layouts) as a ``triples.restart`` file (ccx_write_restart)."""
+    eh = _host(inp["eh"])
+    no, nu = eh.shape[0], _host(inp["ep"]).shape[0]
+    arrs = [_host(inp[k]) for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")]
+    _check(lib().ccx_write_restart(str(path).encode(), no, nu, *[a.ctypes.data_as(_dp) for a in arrs],
+                                   1 if packed_vvvo else 0), "ccx_write_restart")
 
 
 def algorithmic_flops(no: int, nu: int) -> float:

@@ -81,3 +81,12 @@ def cube_tables(n: int) -> np.ndarray:
            "vvvo": np.zeros((n, n, n, 1))}
     o = run(inp, 5)
     return o.reshape(13, n, n, n).transpose(0, 1, 2, 3).reshape(13, n * n * n)
+
+
+def tranmd_table(shape: tuple, code: int) -> np.ndarray:
+    """a(n1,n2,n3,n4) filled with 1..n1*n2*n3*n4 after the reference's TRANMD_SMP(a,...,code)
+    (harness mode 7), as the flat column-major array."""
+    inp = {"no": 1, "nu": 1, "eh": np.zeros(1), "ep": np.zeros(1), "t1": np.zeros(1),
+           "t2": np.zeros(1), "vm": np.zeros(1), "voe": np.zeros(1), "vvvo": np.zeros((1, 1, 1, 1))}
+    n1, n2, n3, n4 = shape
+    return run(inp, 7, n1, n2, n3, lo=n4, hi=code)

@@ -22,6 +22,8 @@
 !            TRANT3_ACC ids 1..5, tranmd_23, SYMT311_ACC 23/12/13 (13 cubes)
 !   mode 6 : VE for occupied index i from the packed patch: DDCC_T_GETVE_acc
 !            then trant3_1; out: ve(nu^3)
+!   mode 7 : the array a(i,j,k,lo) filled with 1..i*j*k*lo after TRANMD_SMP with
+!            code hi; out: a
 program ref_harness
 use common_cc
 implicit none
@@ -165,6 +167,14 @@ case (6)
    call getve(i, ve_i)
    call trant3_1(nu, ve_i)
    write(11) ve_i
+case (7)
+   n = i*j*k*lo
+   allocate(cube(n))
+   do iwrk = 1, n
+      cube(iwrk) = dble(iwrk)
+   end do
+   call tranmd_smp(cube, i, j, k, lo, hi)
+   write(11) cube
 end select
 close(11)
 

@@ -13,7 +13,8 @@ the generator is caught):
   tuples of each kind on the (4,6,11) case;
 * the 13 cube-operation tables (trant3_1/3/4/5, TRANT3_ACC 1..5, tranmd_23,
   SYMT311_ACC 23/12/13) on the 2x2x2 and 3x3x3 cubes filled with 1..n^3, the
-  known-answer table that the reference's tools/trant3 program prints.
+  known-answer table that the reference's tools/trant3 program prints;
+* TRANMD_SMP's output for every code on 4-index arrays filled with 1..n.
 """
 
 from __future__ import annotations
@@ -33,6 +34,12 @@ CASES = [(3, 5, 7), (4, 6, 11), (5, 7, 3), (2, 9, 5), (6, 4, 21),
          (4, 36, 1001), (4, 87, 1002), (8, 120, 1003)]
 TUPLES = [(0, 2, 1, 0), (0, 3, 2, 1), (0, 3, 1, 0), (1, 2, 1, 1), (1, 3, 0, 0),
           (2, 2, 2, 1), (2, 3, 3, 0)]
+# TRANMD_SMP codes on the 3^4 array and on shapes where only the permuted extents agree
+CODES = (12, 13, 14, 23, 24, 34, 231, 312, 341, 413, 1234, 99)
+TRANMD = [((3, 3, 3, 3), c) for c in CODES] + [
+    ((3, 3, 2, 4), 12), ((3, 2, 3, 4), 13), ((3, 2, 4, 3), 14), ((2, 3, 3, 4), 23), ((2, 3, 4, 3), 24),
+    ((2, 4, 3, 3), 34), ((3, 3, 3, 2), 231), ((3, 3, 3, 2), 312), ((3, 2, 3, 3), 341), ((3, 2, 3, 3), 413),
+    ((3, 2, 2, 3), 1234), ((2, 3, 4, 5), 99)]
 
 
 def checksum(inp: dict) -> np.ndarray:
@@ -59,6 +66,8 @@ def main() -> None:
         out[f"tetd_{tag}"] = np.array([t["etd"]])
     for n in (2, 3):
         out[f"cube_{n}"] = ref.cube_tables(n)
+    for shape, code in TRANMD:
+        out["tranmd_%d_%d_%d_%d_%d" % (shape + (code,))] = ref.tranmd_table(shape, code)
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **out)
 
 

@@ -1,5 +1,6 @@
 """The C-ABI library: it loads, exports every function include/libcc_mi355x.h declares,
-its host-only legacy entries (trant3 family, tranmd_23) match the reference tables, and
+its host-only legacy entries (trant3 family, tranmd_23, TRANMD_SMP) match the reference
+tables, and
 its argument / state checks return the documented status codes.  No GPU needed."""
 
 import ctypes
@@ -74,6 +75,58 @@ def test_tranmd_23(golden):
     assert np.array_equal(a, want)
 
 
+def _tranmd_cases():
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import TRANMD
+    return TRANMD
+
+
+@pytest.mark.parametrize("shape,code", _tranmd_cases(), ids=lambda v: str(v))
+def test_tranmd_smp_matches_reference(golden, shape, code):
+    """Every TRANMD_SMP code, on the 3^4 array and on shapes where only the permuted extents
+    agree, against the reference routine's own output (golden vectors)."""
+    a = np.arange(1, np.prod(shape) + 1, dtype=float)
+    lib().tranmd_smp_(_dp(a), *[_i64(v) for v in shape], _i64(code))
+    assert lib().ccx_last_status() == 0
+    assert np.array_equal(a, golden["tranmd_%d_%d_%d_%d_%d" % (tuple(shape) + (code,))])
+
+
+def test_tranmd_smp_rejects_unequal_extents():
+    a = np.arange(1, 2 * 3 * 4 * 5 + 1, dtype=float)
+    before = a.copy()
+    lib().tranmd_smp_(_dp(a), _i64(2), _i64(3), _i64(4), _i64(5), _i64(12))
+    assert lib().ccx_last_status() == 1
+    assert np.array_equal(a, before)
+
+
+def test_async_and_queue_variants_equal_plain():
+    rng = np.random.default_rng(5)
+    for plain, asy in (("trant3_1_", "trant3_1_async_"), ("trant3_4_", "trant3_4_async_")):
+        c = np.asfortranarray(rng.normal(size=(5, 5, 5)))
+        d = c.copy(order="F")
+        getattr(lib(), plain)(_i64(5), _dp(c))
+        getattr(lib(), asy)(_i64(3), _i64(5), _dp(d))
+        assert lib().ccx_last_status() == 0
+        assert np.array_equal(c, d)
+    a = np.asfortranarray(rng.normal(size=(2, 4, 4, 3)))
+    b = a.copy(order="F")
+    lib().tranmd_23_(_dp(a), _i64(2), _i64(4), _i64(4), _i64(3))
+    lib().tranmd_23_acc_(_i64(1), _dp(b), _i64(2), _i64(4), _i64(4), _i64(3))
+    assert np.array_equal(a, b)
+
+
+def test_gpu_trio_order_errors():
+    """ijk_gpu_driver_ / ijk_gpu_finalize_ before ijk_gpu_init_ report a state error."""
+    L = lib()
+    z = np.zeros(8)
+    L.ijk_gpu_finalize_(_i64(2), _i64(2), _dp(z))
+    assert L.ccx_last_status() == 4
+    L.ijk_gpu_driver_(_i64(2), _i64(2), _i64(1), _i64(1), _i64(1), _dp(z), _dp(z), _dp(z), _dp(z), _dp(z), _dp(z),
+                      _dp(z))
+    assert L.ccx_last_status() == 4
+
+
 def test_trant3_random_cube_matches_port():
     rng = np.random.default_rng(2)
     for op in range(1, 6):
@@ -96,7 +149,7 @@ def test_argument_errors():
     assert L.ccx_last_status() == 1
     L.common_cc_init_(_i64(0), _i64(5))
     assert L.ccx_last_status() == 1
-    for s in range(7):
+    for s in range(8):
         assert L.ccx_error_string(s)
 
 

@@ -271,3 +271,96 @@ def test_legacy_tuple_loop_equals_driver(golden):
     ets_want, etd_want = golden["e_5_7_3"]
     assert abs(L.ccx_get_etd_() - etd_want) <= max(1e-9, 1e-12 * abs(etd_want))
     _v1_ok(v1, golden["v1_5_7_3"])
+
+
+def test_abc_per_tuple_entry_equals_ijk_entry():
+    """ddcc_t_ijk_big_ (the triples-abc argument order) adds the same etd and v1 as
+    ddcc_t_ijk_acc_ for the same tuple."""
+    no, nu, seed = 4, 6, 11
+    inp = synth.make_inputs(no, nu, seed)
+    L = lib()
+    f = np.asfortranarray
+    a = {k: f(inp[k].copy()) for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    i, j, k = 3, 2, 0
+    ves = [_ve(inp["vvvo"], x) for x in (i, j, k)]
+    out = []
+    for abc in (False, True):
+        L.common_cc_init_(_i64(no), _i64(nu))
+        v1 = np.zeros((nu, no), order="F")
+        v3 = np.zeros((nu, nu, nu), order="F")
+        if abc:
+            t3, tmp = np.zeros(nu ** 3), np.zeros(nu * nu)
+            L.ddcc_t_ijk_big_(_i64(no), _i64(nu), _i64(i + 1), _i64(j + 1), _i64(k + 1), _dp(v1), _dp(a["t2"]),
+                              _dp(a["vm"]), _dp(v3), _dp(t3), _dp(a["voe"]), _dp(a["t1"]), _dp(tmp), _dp(a["eh"]),
+                              _dp(a["ep"]), *[_dp(v) for v in ves])
+        else:
+            L.ddcc_t_ijk_acc_(_i64(no), _i64(nu), _i64(i + 1), _i64(j + 1), _i64(k + 1), _dp(v1), _dp(a["t2"]),
+                              _dp(a["vm"]), _dp(v3), _dp(a["voe"]), _dp(a["eh"]), _dp(a["ep"]),
+                              *[_dp(v) for v in ves])
+        assert L.ccx_last_status() == 0
+        out.append((L.ccx_get_etd_(), v1, v3))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+def test_cuda_driver_trio_loop_equals_driver(golden):
+    """ijk_gpu_init_ / ijk_gpu_driver_ over every distinct tuple (raw packed VVVO patches, as
+    the reference's CPU driver fetches them) / ijk_gpu_finalize_, then the degenerate tuples
+    through ddcc_t_ijj_acc_ / ddcc_t_iij_acc_: the driver's etd and v1."""
+    no, nu, seed = 5, 7, 3
+    inp = synth.make_inputs(no, nu, seed)
+    L = lib()
+    f = np.asfortranarray
+    a = {k: f(inp[k].copy()) for k in ("eh", "ep", "t1", "t2", "vm", "voe")}
+    packed = synth.pack_vvvo(inp["vvvo"])          # (nutr, nu*no), column c + nu*x
+    patch = {x: f(packed[:, nu * x:nu * (x + 1)]) for x in range(no)}
+    v1 = np.zeros((nu, no), order="F")
+    etd = np.zeros(1)
+    dummy = np.zeros(1)
+    L.common_cc_init_(_i64(no), _i64(nu))
+    L.ijk_gpu_init_(_i64(no), _i64(nu), _dp(a["eh"]), _dp(a["ep"]), _dp(v1), _dp(a["t2"]), _dp(a["vm"]),
+                    _dp(a["voe"]))
+    assert L.ccx_last_status() == 0
+    for i in range(no):
+        for j in range(i):
+            for k in range(j):
+                L.ijk_gpu_driver_(_i64(nu), _i64(no), _i64(i + 1), _i64(j + 1), _i64(k + 1), _dp(patch[i]),
+                                  _dp(patch[j]), _dp(patch[k]), _dp(v1), _dp(a["eh"]), _dp(a["ep"]), _dp(dummy))
+                assert L.ccx_last_status() == 0
+    assert np.all(v1 == 0.0) and etd[0] == 0.0      # nothing lands before finalize
+    L.ijk_gpu_finalize_(_i64(no), _i64(nu), _dp(etd))
+    assert L.ccx_last_status() == 0
+    assert L.ccx_get_etd_() == 0.0                  # the trio writes the caller's etd only
+    ve = {x: _ve(inp["vvvo"], x) for x in range(no)}
+    v3 = np.zeros((nu, nu, nu), order="F")
+    s = [np.zeros(nu ** 3) for _ in range(2)]
+    for i in range(no):
+        for j in range(i):
+            for fn in (L.ddcc_t_ijj_acc_, L.ddcc_t_iij_acc_):
+                fn(_i64(no), _i64(nu), _i64(i + 1), _i64(j + 1), _dp(v1), _dp(a["t2"]), _dp(a["vm"]), _dp(v3),
+                   _dp(s[0]), _dp(a["voe"]), _dp(a["t1"]), _dp(a["eh"]), _dp(a["ep"]), _dp(s[1]), _dp(ve[i]),
+                   _dp(ve[j]))
+    ets_want, etd_want = golden["e_5_7_3"]
+    total = etd[0] + L.ccx_get_etd_()
+    assert abs(total - etd_want) <= max(1e-9, 1e-12 * abs(etd_want))
+    _v1_ok(v1, golden["v1_5_7_3"])
+
+
+def test_restart_file_loads_same_energy(golden, tmp_path):
+    """A triples.restart written from the inputs loads (ccx_load_restart: packed VVVO expanded
+    on the device) to the same energies, bitwise, and to the reference's golden energy."""
+    from libcc_amd.triples import write_restart
+
+    no, nu, seed = 4, 36, 1001
+    inp = synth.make_inputs(no, nu, seed)
+    path = str(tmp_path / "triples.restart")
+    write_restart(path, inp)
+    a = TriplesContext(no, nu)
+    a.load(inp)
+    ra = a.energy()
+    b = TriplesContext.from_restart(path)
+    rb = b.energy()
+    assert ra["etd"] == rb["etd"] and ra["ets"] == rb["ets"]
+    _energy_ok(rb["ets"], rb["etd"], *golden["e_4_36_1001"])
+    a.close()
+    b.close()

@@ -38,15 +38,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-PROFILE = os.path.join(ROOT, "profiles", "triples_kernel_fp64_no24_nu216.md")
-
-
-def measured_traffic(config: str):
-    """HBM bytes per launch of the persistent FP64 kernel at no=24, nu=216, as the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary records it (copy-calibrated), else None."""
-    if config != "c4" or not os.path.exists(PROFILE):
+def measured_traffic(config: str, form: str):
+    """Bytes per launch between L2 and the data fabric of the persistent kernel at no=24,
+    nu=216 in this arithmetic form, as the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    records it (copy-calibrated), else None."""
+    prof = os.path.join(ROOT, "profiles", f"triples_kernel_{form}_no24_nu216.md")
+    if config != "c4" or not os.path.exists(prof):
         return None
-    for line in open(PROFILE):
+    for line in open(prof):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
@@ -209,7 +208,7 @@ def main() -> None:
         "kernel_ms": kernel_s * 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": measured_traffic(args.config) if args.form == "fp64" else None,
+                     "traffic": measured_traffic(args.config, args.form),
                      "flops_per_launch": flops, "executed_flops_per_launch": exec_flops, "work_items": items},
         "cpu_baseline": None,
     }

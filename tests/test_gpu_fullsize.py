@@ -1,0 +1,85 @@
+"""Parity at the full no=24, nu=216 shape, where the oracle is too slow to run, through
+properties the mathematics fixes:
+
+* W is linear in (VVVO, VM) at fixed T2, and scaling by 2 is exact in every FP64 product and
+  sum, so doubling VVVO and VM must give E[T] x 4 and E_ST x 2 and v1 x 2 **bitwise**;
+* E(T) is a sum over all occupied tuples and all virtual triples, so relabelling the occupied
+  or the virtual orbitals (consistently in every array) leaves it unchanged up to summation
+  order, and permutes v1 the same way.  Relabelling the virtuals moves every element to
+  other 16-blocks and block orbits, so it exercises the whole block-orbit bookkeeping.
+
+Inputs are the seeded generator's arrays built on the device (libcc_amd.synth)."""
+
+import numpy as np
+import pytest
+
+from libcc_amd import synth
+from libcc_amd.triples import TriplesContext
+
+pytestmark = pytest.mark.gpu
+
+NO, NU, SEED = 24, 216, 1004
+
+
+@pytest.fixture(scope="module")
+def full():
+    import torch
+
+    inp = synth.make_inputs_torch(NO, NU, SEED, "cuda:0")
+    ctx = TriplesContext(NO, NU)
+    ctx.load(inp)
+    ref = ctx.energy()
+    yield torch, inp, ctx, ref
+    ctx.close()
+    del inp
+    torch.cuda.empty_cache()
+
+
+def _c(t, fshape):
+    """C-order view of a Fortran-ordered 1-D buffer: axis 0 is the last Fortran index."""
+    return t.view(tuple(reversed(fshape)))
+
+
+def test_power_of_two_scaling_is_exact(full):
+    torch, inp, ctx, ref = full
+    sc = dict(inp, vvvo=inp["vvvo"] * 2.0, vm=inp["vm"] * 2.0)
+    ctx.load(sc)
+    r = ctx.energy()
+    ctx.load(inp)
+    assert r["etd"] == 4.0 * ref["etd"]
+    assert r["ets"] == 2.0 * ref["ets"]
+    assert np.array_equal(r["v1"], 2.0 * ref["v1"])
+
+
+def _relabelled(torch, inp, p_occ, p_vir):
+    no, nu = NO, NU
+    po = torch.as_tensor(p_occ, device="cuda:0")
+    pv = torch.as_tensor(p_vir, device="cuda:0")
+    out = {"eh": inp["eh"][po].contiguous(), "ep": inp["ep"][pv].contiguous()}
+    t1 = _c(inp["t1"], (no, nu))[pv][:, po]                                   # (nu, no)
+    t2 = _c(inp["t2"], (nu, nu, no, no))[po][:, po][:, :, pv][:, :, :, pv]    # (no, no, nu, nu)
+    vm = _c(inp["vm"], (no, nu, no, no))[po][:, po][:, :, pv][:, :, :, po]    # (no, no, nu, no)
+    voe = _c(inp["voe"], (nu, nu, no, no))[po][:, po][:, :, pv][:, :, :, pv]  # (no, no, nu, nu)
+    vvvo = _c(inp["vvvo"], (nu, nu, nu, no))[po][:, pv][:, :, pv][:, :, :, pv]  # (no, nu, nu, nu)
+    for k, v in (("t1", t1), ("t2", t2), ("vm", vm), ("voe", voe), ("vvvo", vvvo)):
+        out[k] = v.contiguous().reshape(-1)
+    return out
+
+
+@pytest.mark.parametrize("which", ["occupied", "virtual"])
+def test_relabelling_invariance(full, which):
+    torch, inp, ctx, ref = full
+    rng = np.random.default_rng(7)
+    p_occ = rng.permutation(NO) if which == "occupied" else np.arange(NO)
+    p_vir = rng.permutation(NU) if which == "virtual" else np.arange(NU)
+    rel = _relabelled(torch, inp, p_occ, p_vir)
+    ctx.load(rel)
+    r = ctx.energy()
+    ctx.load(inp)
+    del rel
+    scale = abs(ref["etd"])
+    assert abs(r["etd"] - ref["etd"]) <= 1e-11 * scale, (r["etd"], ref["etd"])
+    assert abs(r["ets"] - ref["ets"]) <= 1e-11 * scale, (r["ets"], ref["ets"])
+    # v1(a, i) of the relabelled problem = v1(p_vir[a], p_occ[i]) of the original
+    want = ref["v1"][np.ix_(p_vir, p_occ)]
+    np.testing.assert_allclose(r["v1"], want, rtol=0, atol=1e-10 * np.abs(want).max())

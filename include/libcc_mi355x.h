@@ -98,6 +98,13 @@ int ccx_work_stats(const ccx_ctx *ctx, int64_t *items, double *executed_flops);
 int ccx_gradient(ccx_ctx *ctx, double *ets, double *etd, double *dt1, double *dt2, double *dvm, double *dvoe,
                  double *dvvvo, int on_device);
 
+/* ccx_gradient with the arithmetic form of the per-tuple W recomputation: CCX_FORM_FP64 (as
+ * ccx_gradient) or CCX_FORM_BF16X3 (W from the split BF16 slices with FP32 accumulation, then
+ * FP64: dE/dW, every back-contraction and every gradient accumulation stay FP64).  Other
+ * forms return CCX_ERR_FORM. */
+int ccx_gradient_form(ccx_ctx *ctx, int form, double *ets, double *etd, double *dt1, double *dt2, double *dvm,
+                      double *dvoe, double *dvvvo, int on_device);
+
 /* Set the stream (hipStream_t cast to void*) later calls use; NULL = default stream. */
 int ccx_set_stream(ccx_ctx *ctx, void *stream);
 

@@ -60,6 +60,8 @@ def lib() -> ctypes.CDLL:
     L.ccx_work_stats.restype = ctypes.c_int
     L.ccx_gradient.argtypes = [ctxp, dp, dp] + [vp] * 5 + [ctypes.c_int]
     L.ccx_gradient.restype = ctypes.c_int
+    L.ccx_gradient_form.argtypes = [ctxp, ctypes.c_int, dp, dp] + [vp] * 5 + [ctypes.c_int]
+    L.ccx_gradient_form.restype = ctypes.c_int
     L.ccx_set_stream.argtypes = [ctxp, vp]
     L.ccx_set_stream.restype = ctypes.c_int
     pi64 = ctypes.POINTER(i64)

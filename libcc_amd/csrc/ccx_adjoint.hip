@@ -344,6 +344,10 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
 
 }  // namespace
 
+// SPLIT: W is recomputed by the split BF16x3 K sweep (FP32 accumulation, layout row
+// 4 (lane >> 4) + v) and staged, scaled to FP64, in the FP64 layout (row (lane >> 4) + 4 v) that
+// the epilogue reads; everything after the staging is the FP64 path.
+template <bool SPLIT>
 __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
 {
     extern __shared__ double lds[];
@@ -365,20 +369,41 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
         const long long item = (long long)t * g.nblk + q;
         int tkind, mult;
         const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
-        d4 acc[6][BPW];
-        if (tkind == 2) item_w<M_IJJ, double>(g, lds, it, acc, wave, lane);
-        else if (tkind == 3) item_w<M_IIJ, double>(g, lds, it, acc, wave, lane);
-        else if (mult == 1) item_w<M_FULL, double>(g, lds, it, acc, wave, lane);
-        else if (mult == 6) item_w<M_AAA, double>(g, lds, it, acc, wave, lane);
-        else if (it.blk[0] == it.blk[1]) item_w<M_AAC, double>(g, lds, it, acc, wave, lane);
-        else item_w<M_ACC, double>(g, lds, it, acc, wave, lane);
         double *ws = ga.wstage + (long long)blockIdx.x * (6 * BPW * 4 * NTHR);
+        if constexpr (SPLIT) {
+            f4 acc[6][BPW];
+            if (tkind == 2) item_w_split<M_IJJ>(g, lds, it, acc, wave, lane);
+            else if (tkind == 3) item_w_split<M_IIJ>(g, lds, it, acc, wave, lane);
+            else if (mult == 1) item_w_split<M_FULL>(g, lds, it, acc, wave, lane);
+            else if (mult == 6) item_w_split<M_AAA>(g, lds, it, acc, wave, lane);
+            else if (it.blk[0] == it.blk[1]) item_w_split<M_AAC>(g, lds, it, acc, wave, lane);
+            else item_w_split<M_ACC>(g, lds, it, acc, wave, lane);
+            // element (row 4 (lane >> 4) + v, column lane & 15) belongs to FP64-layout lane
+            // 16 v + (lane & 15), slot lane >> 4 of the same wave
+            const int dst = wave * 64 + (lane & 15);
 #pragma unroll
-        for (int p = 0; p < 6; ++p)
+            for (int p = 0; p < 6; ++p)
 #pragma unroll
-            for (int bb = 0; bb < BPW; ++bb)
+                for (int bb = 0; bb < BPW; ++bb)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) ws[((p * BPW + bb) * 4 + v) * NTHR + tid] = acc[p][bb][v];
+                    for (int v = 0; v < 4; ++v)
+                        ws[((p * BPW + bb) * 4 + (lane >> 4)) * NTHR + dst + 16 * v] = (double)acc[p][bb][v] * g.wscale;
+            __syncthreads();
+        } else {
+            d4 acc[6][BPW];
+            if (tkind == 2) item_w<M_IJJ, double>(g, lds, it, acc, wave, lane);
+            else if (tkind == 3) item_w<M_IIJ, double>(g, lds, it, acc, wave, lane);
+            else if (mult == 1) item_w<M_FULL, double>(g, lds, it, acc, wave, lane);
+            else if (mult == 6) item_w<M_AAA, double>(g, lds, it, acc, wave, lane);
+            else if (it.blk[0] == it.blk[1]) item_w<M_AAC, double>(g, lds, it, acc, wave, lane);
+            else item_w<M_ACC, double>(g, lds, it, acc, wave, lane);
+#pragma unroll
+            for (int p = 0; p < 6; ++p)
+#pragma unroll
+                for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) ws[((p * BPW + bb) * 4 + v) * NTHR + tid] = acc[p][bb][v];
+        }
         adj_epilogue(ga, lds, red, it, item, tl, tkind, mult, ws, wave, lane, tid);
     }
 }
@@ -801,15 +826,18 @@ size_t ccx_hip_adj_stage_doubles(void) { return (size_t)6 * BPW * 4 * NTHR; }
 int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const double *voep, const double *eh,
                       const double *ep, const double *t1, const int *tuples, const int *blocks, double *e_part,
                       double *v1_part, double *wbar, double *slots, double *wstage, unsigned int *counter, int no,
-                      int nu, int nup, int kp, int ntup, int nblk, int t0, int tcount, int grid, void *stream)
+                      int nu, int nup, int kp, int ntup, int nblk, int t0, int tcount, int grid,
+                      const void *const *split, int kps, double wscale, void *stream)
 {
-    static bool attr_set = false;
+    static bool attr_set[2] = {false, false};
     const size_t lds_bytes = sizeof(double) * ADJ_LDS;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)adj_item_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_bytes) != hipSuccess)
+    const int sp = split != nullptr;
+    if (sp && kps % 64 != 0) return 3;
+    if (!attr_set[sp]) {
+        const void *fn = sp ? (const void *)adj_item_kernel<true> : (const void *)adj_item_kernel<false>;
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return 2;
-        attr_set = true;
+        attr_set[sp] = true;
     }
     AdjArgs ga;
     TriplesArgs &g = ga.f;
@@ -818,6 +846,11 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
     g.block_major = 0;
+    g.axr_s = sp ? split[0] : nullptr;
+    g.axc_s = sp ? split[1] : nullptr;
+    g.bx_s = sp ? split[2] : nullptr;
+    g.kps = kps;
+    g.wscale = sp ? wscale : 1.0;
     ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.wstage = wstage; ga.t0 = t0; ga.tcount = tcount;
     const long long nlocal = (long long)tcount * nblk;
     if (nlocal == 0) return 0;
@@ -825,7 +858,8 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     if (grid > nlocal) grid = (int)nlocal;
     hipStream_t s = (hipStream_t)stream;
     if (hipMemsetAsync(counter, 0, sizeof(unsigned int), s) != hipSuccess) return 1;
-    adj_item_kernel<<<grid, NTHR, lds_bytes, s>>>(ga);
+    if (sp) adj_item_kernel<true><<<grid, NTHR, lds_bytes, s>>>(ga);
+    else adj_item_kernel<false><<<grid, NTHR, lds_bytes, s>>>(ga);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -490,8 +490,20 @@ static int build_csr(int nkeys, int n, const int *key, const int *tl, const int 
 int ccx_gradient(ccx_ctx *c, double *ets, double *etd, double *dt1, double *dt2, double *dvm, double *dvoe,
                  double *dvvvo, int on_device)
 {
+    return ccx_gradient_form(c, CCX_FORM_FP64, ets, etd, dt1, dt2, dvm, dvoe, dvvvo, on_device);
+}
+
+int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt1, double *dt2, double *dvm,
+                      double *dvoe, double *dvvvo, int on_device)
+{
     if (!c) return CCX_ERR_ARG;
     if (!c->loaded) return CCX_ERR_STATE;
+    if (form != CCX_FORM_FP64 && form != CCX_FORM_BF16X3) return CCX_ERR_FORM;
+    if (form == CCX_FORM_BF16X3) {
+        int rs = ensure_split(c);
+        if (rs != CCX_OK) return rs;
+    }
+    const void *split[3] = {c->d_axr_s, c->d_axc_s, c->d_bx_s};
     const int64_t no = c->no, nu = c->nu, nup = c->nup, nb = c->nb, kp = c->kp;
     const int64_t ntup = c->ntup, nblk = c->nblk;
     int rc = CCX_OK;
@@ -567,7 +579,8 @@ int ccx_gradient(ccx_ctx *c, double *ets, double *etd, double *dt1, double *dt2,
                               c->d_blocks, c->d_epart, c->d_v1part, d_wbar, d_slots, d_wstage, c->d_counter, (int)no,
                               (int)nu,
                               (int)nup, (int)kp, (int)ntup, (int)nblk, (int)t0, tcount,
-                              grid_size(c, (int64_t)tcount * nblk), c->stream) != 0 ||
+                              grid_size(c, (int64_t)tcount * nblk), form == CCX_FORM_BF16X3 ? split : NULL,
+                              (int)c->kps, c->split_scale, c->stream) != 0 ||
             ccx_hip_gax(d_wbar, c->d_bx, d_gax, c->d_tuples, (int)t0, dl[0][0], dl[0][1], dl[0][2], nkeys[0], (int)no,
                         (int)nup, (int)kp, c->stream) != 0 ||
             ccx_hip_gbx(d_wbar, c->d_axr, c->d_axc, d_gbx, d_gpart, c->d_tuples, (int)t0, dl[1][0], dl[1][1],

@@ -11,6 +11,7 @@
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
@@ -442,6 +443,170 @@ __device__ __forceinline__ void item_w(const TriplesArgs &g, double *lds, const 
     run_phase<2, MODE, T>(g, it, acc, wave, lane);
     complete_orderings<MODE, T>(lds, acc, wave, lane);
 }
+
+// ---- split BF16 K sweep -----------------------------------------------------------
+// n-th set bit of a mask (compile time)
+__host__ __device__ constexpr int nth_bit(int mask, int n)
+{
+    for (int b = 0; b < 32; ++b)
+        if ((mask >> b) & 1) {
+            if (n == 0) return b;
+            --n;
+        }
+    return -1;
+}
+
+// Terms of one phase that read AX slot s: ordering p and BX operand b of each.
+struct SlotTerms {
+    int n;
+    int p[12], b[12];
+};
+__host__ __device__ constexpr SlotTerms slot_terms(int phase, int mode, int s)
+{
+    SlotTerms r{0, {}, {}};
+    const int mm = mode_mask(mode);
+    for (int p = 0; p < 6; ++p)
+        if ((mm >> p) & 1) {
+            const Term t = term(phase, p);
+            if (t.s1 == s) { r.p[r.n] = p; r.b[r.n] = t.b1; ++r.n; }
+            if (t.s2 == s) { r.p[r.n] = p; r.b[r.n] = t.b2; ++r.n; }
+        }
+    return r;
+}
+
+typedef const __attribute__((address_space(1))) bf16x8 gbf8;   // global memory, 16 B per lane
+
+// Blocks of 32 K: the BX slices of the phase (shared by all waves) are staged in LDS, double
+// buffered across blocks; the AX slices stream per operand slot through a two-deep register
+// ring (two blocks per loop iteration keep the ring index a compile-time constant).  Each
+// slot step reads the BX fragments of its terms from LDS first, then issues the 6 slice
+// products i + j <= 2 of every term and batch, smallest first, with consecutive MFMAs on
+// different accumulators.  Loads are global-address-space with one uniform base per operand,
+// so they count only against the vector-memory counter and never delay an LDS wait.
+template <int PHASE, int MODE>
+__device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const ItemCtx &it, f4 (&acc)[6][BPW],
+                                                __bf16 *lbx, int wave, int lane)
+{
+    constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
+    constexpr int NSL = __builtin_popcount(SM), NBX = __builtin_popcount(XM);
+    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
+    int r0[2], r1[2];
+    bool col[2];
+    if (PHASE == 0) { r0[0] = A; r1[0] = C; col[0] = true;  r0[1] = C; r1[1] = A; col[1] = false; }
+    else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
+    else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
+    const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
+    const int nkb = g.kps / 32;
+    // units of 16 B: an AX panel is 16 outer rows x nkb blocks x 3 slices x 64 lanes = 96 kps,
+    // a BX panel nkb x 3 x 64 = 6 kps; this wave's outer rows are 2 wave and 2 wave + 1
+    gbf8 *ap[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int o = s / 3, xp = s % 3;
+        const long long panel = ((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o];
+        ap[s] = (gbf8 *)(col[o] ? g.axc_s : g.axr_s) + panel * (96LL * g.kps) +
+                (long long)wave * (2 * nkb * 192);
+    }
+    gbf8 *xq[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        xq[q] = (gbf8 *)g.bx_s + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (6LL * g.kps);
+    const int lo = lane * 8;
+
+    // BX staging: chunk c = rank * 3 + slice (64 lanes x 16 B) is loaded by wave c % NWAVE
+    bf16x8 br[3];
+    auto bx_load = [&](int kb) __attribute__((always_inline)) {
+        static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(Cc)::value;
+            constexpr int q = nth_bit(XM, c / 3), sl = c % 3;
+            if (wave == c % NWAVE) br[c / NWAVE] = xq[q][(kb * 3 + sl) * 64 + lane];
+        });
+    };
+    auto bx_store = [&](int buf) __attribute__((always_inline)) {
+        static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(Cc)::value;
+            if (wave == c % NWAVE) *reinterpret_cast<bf16x8 *>(lbx + ((buf * NBX) * 3 + c) * 512 + lo) = br[c / NWAVE];
+        });
+    };
+    bf16x8 ab[2][BPW][3];
+    auto a_load = [&](int s, int kb, bf16x8 (&d)[BPW][3]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb)
+#pragma unroll
+            for (int sl = 0; sl < 3; ++sl) d[bb][sl] = ap[s][((bb * nkb + kb) * 3 + sl) * 64 + lane];
+    };
+    auto compute = [&](auto Sc, const bf16x8 (&a)[BPW][3], int buf) __attribute__((always_inline)) {
+        constexpr int s = decltype(Sc)::value;
+        constexpr SlotTerms ST = slot_terms(PHASE, MODE, s);
+        bf16x8 bs[ST.n > 0 ? ST.n : 1][3];
+        static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+            constexpr int n = decltype(N)::value;
+            constexpr int rank = __builtin_popcount(XM & ((1 << ST.b[n]) - 1));
+#pragma unroll
+            for (int sl = 0; sl < 3; ++sl)
+                bs[n][sl] = *reinterpret_cast<const bf16x8 *>(lbx + ((buf * NBX + rank) * 3 + sl) * 512 + lo);
+        });
+        // slice products with i + j <= 2, smallest first
+        static_for<6>([&](auto Q) __attribute__((always_inline)) {
+            constexpr int qq = 5 - decltype(Q)::value;
+            constexpr int si = qq == 0 ? 0 : qq == 1 ? 0 : qq == 2 ? 1 : qq == 3 ? 0 : qq == 4 ? 1 : 2;
+            constexpr int sj = qq == 0 ? 0 : qq == 1 ? 1 : qq == 2 ? 0 : qq == 3 ? 2 : qq == 4 ? 1 : 0;
+            static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+                constexpr int n = decltype(N)::value;
+                constexpr int p = ST.p[n];
+#pragma unroll
+                for (int bb = 0; bb < BPW; ++bb) {
+                    if constexpr (PHASE == 2)
+                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bs[n][sj], a[bb][si], acc[p][bb], 0, 0, 0);
+                    else
+                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[bb][si], bs[n][sj], acc[p][bb], 0, 0, 0);
+                }
+            });
+        });
+    };
+
+    bx_load(0);
+    bx_store(0);
+    __syncthreads();
+    a_load(nth_bit(SM, 0), 0, ab[0]);
+    for (int kb = 0; kb < nkb; kb += 2) {
+        static_for<2>([&](auto HB) __attribute__((always_inline)) {
+            constexpr int hb = decltype(HB)::value;
+            const int kbb = kb + hb;
+            bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+            static_for<NSL>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                constexpr int t = hb * NSL + j;
+                constexpr int s = nth_bit(SM, j);
+                constexpr int sn = nth_bit(SM, (j + 1) % NSL);
+                const int kn = j + 1 < NSL ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+                a_load(sn, kn, ab[(t + 1) % 2]);
+                __builtin_amdgcn_sched_barrier(0);
+                compute(std::integral_constant<int, s>{}, ab[t % 2], hb);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            bx_store(1 - hb);
+            __syncthreads();
+        });
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void item_w_split(const TriplesArgs &g, double *lds, const ItemCtx &it,
+                                             f4 (&acc)[6][BPW], int wave, int lane)
+{
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) acc[p][bb] = f4{0, 0, 0, 0};
+    __bf16 *lbx = reinterpret_cast<__bf16 *>(lds);
+    run_phase_split<0, MODE>(g, it, acc, lbx, wave, lane);
+    transpose_lb_to_l<MODE, float>(lds, acc, wave, lane);
+    run_phase_split<1, MODE>(g, it, acc, lbx, wave, lane);
+    run_phase_split<2, MODE>(g, it, acc, lbx, wave, lane);
+    complete_orderings<MODE, float>(lds, acc, wave, lane);
+}
+
 
 }  // namespace
 

@@ -82,8 +82,9 @@ class TriplesContext:
                                     ctypes.c_void_p(v1.ctypes.data), 0), "ccx_energy")
         return {"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value, "v1": v1}
 
-    def gradient(self, device: bool = False) -> dict:
-        """E(T) and dE/d(t1, t2, vm, voe, vvvo) in the reference layouts (FP64, ccx_gradient).
+    def gradient(self, device: bool = False, form: str = "fp64") -> dict:
+        """E(T) and dE/d(t1, t2, vm, voe, vvvo) in the reference layouts (ccx_gradient_form; form
+        "fp64" or "bf16x3" for the W recomputation, everything after it FP64).
 
         With ``device=True`` the gradients are torch float64 tensors on the current GPU,
         written in place by the library; otherwise Fortran-ordered numpy arrays."""
@@ -100,8 +101,9 @@ class TriplesContext:
             out = {k: np.zeros(v, order="F") for k, v in shapes.items()}
             ptrs = {k: ctypes.c_void_p(a.ctypes.data) for k, a in out.items()}
         ets, etd = ctypes.c_double(), ctypes.c_double()
-        _check(lib().ccx_gradient(self._ctx, ctypes.byref(ets), ctypes.byref(etd), ptrs["t1"], ptrs["t2"],
-                                  ptrs["vm"], ptrs["voe"], ptrs["vvvo"], 1 if device else 0), "ccx_gradient")
+        _check(lib().ccx_gradient_form(self._ctx, FORMS[form], ctypes.byref(ets), ctypes.byref(etd), ptrs["t1"],
+                                       ptrs["t2"], ptrs["vm"], ptrs["voe"], ptrs["vvvo"], 1 if device else 0),
+               "ccx_gradient_form")
         out.update({"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value})
         return out
 

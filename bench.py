@@ -171,10 +171,14 @@ def main() -> None:
     ctx = TriplesContext(no, nu)
     if args.adjoint_only:
         ctx.load(inp)
+        log("adjoint fp64")
         g = ctx.gradient(device=True)
         out = {"adjoint_ms": ctx.timing()[0], "e_t": g["e"], "no": no, "nu": nu}
+        log(f"adjoint fp64 done: {out['adjoint_ms']:.0f} ms")
         if args.form == "bf16x3":
+            log("adjoint bf16x3")
             out["bf16x3"] = split_adjoint(ctx, g)
+            log(f"adjoint bf16x3 done: {out['bf16x3']['ms']:.0f} ms")
         print(json.dumps(out), flush=True)
         ctx.close()
         return

@@ -467,7 +467,8 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
         const int tl = a.ent[2 * e], sg = a.ent[2 * e + 1];
         const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
         const int tpos[3] = {tp.x, tp.y, tp.z};
-        const Sigma S = SIG[sg];
+        const Sigma S = SIG[sg & 7];
+        const double wgt = (sg & 8) ? 2.0 : 1.0;   // a merged pair of a degenerate tuple
         int bc[3];
         bc[S.u] = R0; bc[S.v] = R1; bc[S.wf] = G;
         const double *wsrc = a.wbar + tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096;
@@ -483,7 +484,7 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
             const int Kl = idx >> 4;
             double2 v = {0.0, 0.0};
             if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(bsrc + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
-            rb[2 * k] = v.x; rb[2 * k + 1] = v.y;
+            rb[2 * k] = v.x * wgt; rb[2 * k + 1] = v.y * wgt;
         }
     };
     auto stash = [&](int buf) {
@@ -514,7 +515,7 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
         const int buf = step & 1;
         if (step + 1 < nsteps) fetch(step + 1);
         const int e = e_beg + step / a.nb;
-        const Sigma S = SIG[a.ent[2 * e + 1]];
+        const Sigma S = SIG[a.ent[2 * e + 1] & 7];
         const int str[3] = {WB_STR_A, WB_STR_B, 1};
         const int su = str[S.u], sv = str[S.v], sw = str[S.wf];
         const double *W = WS + buf * WB_BLK;
@@ -598,7 +599,8 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
         const int tl = a.ent[2 * e], sg = a.ent[2 * e + 1];
         const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
         const int tpos[3] = {tp.x, tp.y, tp.z};
-        const Sigma S = SIG[sg];
+        const Sigma S = SIG[sg & 7];
+        const double wgt = (sg & 8) ? 2.0 : 1.0;   // a merged pair of a degenerate tuple
         const int sa = (S.u == 0 || S.v == 0) ? 0 : 1;
         const int hi = sa == 0 ? 1 : 0;
         const int oa = (sa == S.u) ? S.v : S.u;
@@ -630,7 +632,7 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
         for (int k = 0; k < 4; ++k) {
             const int e = tid + k * GTHR;                 // (K, o) of the 128 x 16 slice
             const int Kl = e >> 4, o = e & 15, K = K0 + Kl;
-            ra[k] = K < a.kp ? src[((long long)(K >> 2) * 64 + (K & 3) * 16 + o) * 2] : 0.0;
+            ra[k] = K < a.kp ? src[((long long)(K >> 2) * 64 + (K & 3) * 16 + o) * 2] * wgt : 0.0;
         }
     };
     auto stash = [&](int buf) {

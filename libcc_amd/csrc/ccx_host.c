@@ -452,6 +452,10 @@ int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
 static const int SIGMA_H[6][6] = {{0, 0, 1, 1, 2, 2}, {1, 1, 0, 0, 2, 2}, {2, 2, 1, 1, 0, 0},
                                   {0, 0, 2, 2, 1, 1}, {1, 1, 2, 2, 0, 0}, {2, 2, 0, 0, 1, 1}};
 static const int PERM_H[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+/* the sigma-term each term becomes when two tuple positions are swapped: 1,2 for (i,j,j),
+ * 0,1 for (i,i,j) (each SIGMA_H row with those positions exchanged) */
+static const int SIGMA_PAIR_IJJ[6] = {3, 5, 4, 0, 2, 1};
+static const int SIGMA_PAIR_IIJ[6] = {1, 0, 5, 4, 3, 2};
 
 /* CSR lists of (tl, code) entries grouped by key, keys in increasing order, entries in
  * (tl, code) order: list[u], off[u..u+1] for the nonzero keys only.  Returns #keys. */
@@ -509,6 +513,7 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
     int rc = CCX_OK;
     double *d_gax = NULL, *d_gbx = NULL, *d_dvoep = NULL, *d_wbar = NULL, *d_slots = NULL, *d_wstage = NULL;
     double *d_gpart = NULL;
+    int64_t gpart_units = 0;   /* entries x splits the gbx partial buffer holds */
     double *d_out[5] = {NULL, NULL, NULL, NULL, NULL};
     int *d_lists = NULL, *h_lists = NULL, *h_tmp = NULL;
     double host_out[2] = {0.0, 0.0};
@@ -534,10 +539,16 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
     DALLOC(d_slots, tb * 6 * nb * nb * nb * 256);
     DALLOC(d_lists, lsz);
     DALLOC(d_wstage, (int64_t)grid_size(c, tb * nblk) * (int64_t)ccx_hip_adj_stage_doubles());
-    {   /* gbx partial tiles: the largest entries x splits over the full and the last batch */
-        const int64_t last = ntup > 0 ? ntup - (ntup - 1) / tb * tb : 1;
-        const int64_t u1 = 6 * tb * gbx_nsplit(6 * tb, nb), u2 = 6 * last * gbx_nsplit(6 * last, nb);
-        DALLOC(d_gpart, (int64_t)ccx_hip_gbx_part_doubles((int)nup, (int)kp, (int)(u1 > u2 ? u1 : u2), 1));
+    {   /* gbx partial tiles: the largest entries x splits any batch can need (a batch holds
+         * 1 .. 6 tb gBX entries, fewer than 6 per tuple when degenerate tuples merge pairs,
+         * and the split count grows as the entry count falls) */
+        int64_t umax = 1;
+        for (int64_t n = 1; n <= 6 * tb; ++n) {
+            const int64_t u = n * gbx_nsplit(n, nb);
+            if (u > umax) umax = u;
+        }
+        gpart_units = umax;
+        DALLOC(d_gpart, (int64_t)ccx_hip_gbx_part_doubles((int)nup, (int)kp, (int)umax, 1));
     }
     if (ccx_hip_zero(d_gax, sizeof(double) * (size_t)(no * nup * nup * kp), c->stream) != 0 ||
         ccx_hip_zero(d_gbx, sizeof(double) * (size_t)(no * no * nup * kp), c->stream) != 0 ||
@@ -551,23 +562,37 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
         int *key = h_tmp, *tlv = h_tmp + nent, *code = h_tmp + 2 * nent, *cnt = h_tmp + 3 * nent;
         /* three CSR sets laid out back to back: [list nkey][off nkey+1][ent 2 nent] */
         const int64_t set = nkey + (nkey + 1) + 2 * nent;
-        int nkeys[3];
+        int nkeys[3], nents[3];
         for (int kind = 0; kind < 3; ++kind) {
             int n = 0;
             for (int tl = 0; tl < tcount; ++tl) {
                 const int *tp = c->h_tuples + 4 * (t0 + tl);
+                /* A degenerate tuple's Wbar is symmetric in the two virtual axes of its equal
+                 * positions, so the sigma-terms pair up (positions 1,2 swapped for (i,j,j), 0,1
+                 * for (i,i,j)) into identical back-contractions: keep one of each pair with
+                 * weight 2 (code bit 3) in the gAX / gBX lists. */
+                const int *pair = tp[3] == 2 ? SIGMA_PAIR_IJJ : tp[3] == 3 ? SIGMA_PAIR_IIJ : NULL;
                 for (int q = 0; q < 6; ++q) {
+                    int cq = q;
+                    if (kind < 2 && pair) {
+                        if (pair[q] < q) continue;
+                        cq = q | 8;
+                    }
                     if (kind == 0) key[n] = tp[SIGMA_H[q][0]];                               /* gAX: x */
                     else if (kind == 1) key[n] = tp[SIGMA_H[q][3]] * (int)no + tp[SIGMA_H[q][4]]; /* gBX: (y,z) */
                     else key[n] = tp[PERM_H[q][1]] * (int)no + tp[PERM_H[q][2]];           /* dVOE: (q,r) */
                     tlv[n] = tl;
-                    code[n] = q;
+                    code[n] = cq;
                     ++n;
                 }
             }
             int *base = h_lists + kind * set;
             nkeys[kind] = build_csr((int)nkey, n, key, tlv, code, base, base + nkey, base + nkey + (nkey + 1), cnt);
+            nents[kind] = n;
+            /* the kernels index entries 0 .. n-1 only: the CSR total must be exactly n */
+            if (base[nkey + nkeys[kind]] != n) { rc = CCX_ERR_STATE; goto fail; }
         }
+        if ((int64_t)nents[1] * gbx_nsplit(nents[1], nb) > gpart_units) { rc = CCX_ERR_STATE; goto fail; }
         if (ccx_hip_memcpy_h2d(d_lists, h_lists, sizeof(int) * (size_t)lsz, c->stream) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
         const int *dl[3][3];
         for (int kind = 0; kind < 3; ++kind) {
@@ -584,7 +609,7 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
             ccx_hip_gax(d_wbar, c->d_bx, d_gax, c->d_tuples, (int)t0, dl[0][0], dl[0][1], dl[0][2], nkeys[0], (int)no,
                         (int)nup, (int)kp, c->stream) != 0 ||
             ccx_hip_gbx(d_wbar, c->d_axr, c->d_axc, d_gbx, d_gpart, c->d_tuples, (int)t0, dl[1][0], dl[1][1],
-                        dl[1][2], nkeys[1], 6 * tcount, gbx_nsplit(6 * tcount, nb), (int)no, (int)nup, (int)kp,
+                        dl[1][2], nkeys[1], nents[1], gbx_nsplit(nents[1], nb), (int)no, (int)nup, (int)kp,
                         c->stream) != 0 ||
             ccx_hip_dvoe_reduce(d_slots, d_dvoep, dl[2][0], dl[2][1], dl[2][2], nkeys[2], (int)no, (int)nup,
                                 c->stream) != 0) {

@@ -431,11 +431,15 @@ struct GemmArgs {
     const int *ent;         // (tl, sigma) pairs
     double *part;           // gbx: per (entry, split, chunk, gamma group) partial tiles
     int no, nup, nb, kp, nchunk, gt, ngg, nsplit;
+    int ch0;                // gax: first K chunk of this launch (nchunk chunks from there)
 };
 
 // gAX[x](R0 16 x R1 16 rows, K chunk) += sum over entries (t,sigma) with t[xp] = x and over
 // the gamma blocks G of Wbar_t(e_u = a, e_v = b, e_wf = g) BX[t[yp],t[zp]](g, K).
 // Wave w owns rows b in {2w, 2w+1}, all 8 K tiles of the chunk.
+// NCT: K tiles of 16 the work group computes (GCT for full chunks, fewer for the last chunk
+// when kp is not a multiple of GCH, so no MFMA runs on K padding).
+template <int NCT>
 __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
 {
     extern __shared__ double lds[];
@@ -444,7 +448,7 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int u = blockIdx.x;
-    const int ch = u % a.nchunk; u /= a.nchunk;
+    const int ch = a.ch0 + u % a.nchunk; u /= a.nchunk;
     const int R1 = u % a.nb; u /= a.nb;
     const int R0 = u % a.nb; u /= a.nb;
     const int ui = u;
@@ -454,11 +458,11 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
     const long long cube = (long long)a.nb * a.nb * a.nb * 4096;
     const int nsteps = (e_end - e_beg) * a.nb;
 
-    d4 acc[GBPW][GCT];
+    d4 acc[GBPW][NCT];
 #pragma unroll
     for (int bb = 0; bb < GBPW; ++bb)
 #pragma unroll
-        for (int ct = 0; ct < GCT; ++ct) acc[bb][ct] = d4{0, 0, 0, 0};
+        for (int ct = 0; ct < NCT; ++ct) acc[bb][ct] = d4{0, 0, 0, 0};
 
     // staging registers: Wbar block 4096 = 512 x 8, BX slice 16 x 128 = 512 x 4
     double rw[8], rb[4];
@@ -527,7 +531,7 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
 #pragma unroll
             for (int bb = 0; bb < GBPW; ++bb) af[bb] = W[(lane & 15) * su + (wave * GBPW + bb) * sv + g * sw];
 #pragma unroll
-            for (int ct = 0; ct < GCT; ++ct) {
+            for (int ct = 0; ct < NCT; ++ct) {
                 const double bf = Bs[(ct * 16 + (lane & 15)) * BS_STR + g];
 #pragma unroll
                 for (int bb = 0; bb < GBPW; ++bb) acc[bb][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bb], bf, acc[bb][ct], 0, 0, 0);
@@ -540,7 +544,7 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
 #pragma unroll
     for (int bb = 0; bb < GBPW; ++bb)
 #pragma unroll
-        for (int ct = 0; ct < GCT; ++ct) {
+        for (int ct = 0; ct < NCT; ++ct) {
             const int K = K0 + ct * 16 + (lane & 15);
             if (K >= a.kp) continue;
 #pragma unroll
@@ -804,6 +808,21 @@ __global__ void unpack_grad_kernel(const double *__restrict__ gax, const double 
     }
 }
 
+template <int NCT>
+static int launch_gax(const GemmArgs &a, long long grid, hipStream_t s)
+{
+    static bool attr_set = false;
+    const size_t lds_bytes = sizeof(double) * (2 * WB_BLK + 2 * GCH * BS_STR);
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)gax_kernel<NCT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_bytes) != hipSuccess)
+            return 2;
+        attr_set = true;
+    }
+    gax_kernel<NCT><<<(unsigned)grid, GTHR, lds_bytes, s>>>(a);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 template <int GT>
 static int launch_gbx(const GemmArgs &a, long long grid, hipStream_t s)
 {
@@ -868,22 +887,36 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
 int ccx_hip_gax(const double *wbar, const double *bx, double *gax, const int *tuples, int t0, const int *xlist,
                 const int *xoff, const int *ent, int nx, int no, int nup, int kp, void *stream)
 {
-    static bool attr_set = false;
-    const size_t lds_bytes = sizeof(double) * (2 * WB_BLK + 2 * GCH * BS_STR);
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)gax_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_bytes) != hipSuccess)
-            return 2;
-        attr_set = true;
-    }
     if (nx == 0) return 0;
     GemmArgs a = {};
     a.wbar = wbar; a.bx = bx; a.gax = gax; a.tuples = tuples; a.t0 = t0;
     a.ulist = xlist; a.uoff = xoff; a.ent = ent;
-    a.no = no; a.nup = nup; a.nb = nup / TB; a.kp = kp; a.nchunk = (kp + GCH - 1) / GCH;
-    const long long grid = (long long)nx * a.nb * a.nb * a.nchunk;
-    gax_kernel<<<(unsigned)grid, GTHR, lds_bytes, (hipStream_t)stream>>>(a);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    a.no = no; a.nup = nup; a.nb = nup / TB; a.kp = kp;
+    hipStream_t s = (hipStream_t)stream;
+    // full K chunks, then the last partial chunk with only its K tiles
+    const int nfull = kp / GCH, tail = (kp % GCH + 15) / 16;
+    int rc = 0;
+    if (nfull > 0) {
+        a.ch0 = 0;
+        a.nchunk = nfull;
+        rc = launch_gax<GCT>(a, (long long)nx * a.nb * a.nb * nfull, s);
+    }
+    if (rc == 0 && tail > 0) {
+        a.ch0 = nfull;
+        a.nchunk = 1;
+        const long long grid = (long long)nx * a.nb * a.nb;
+        switch (tail) {
+        case 1: rc = launch_gax<1>(a, grid, s); break;
+        case 2: rc = launch_gax<2>(a, grid, s); break;
+        case 3: rc = launch_gax<3>(a, grid, s); break;
+        case 4: rc = launch_gax<4>(a, grid, s); break;
+        case 5: rc = launch_gax<5>(a, grid, s); break;
+        case 6: rc = launch_gax<6>(a, grid, s); break;
+        case 7: rc = launch_gax<7>(a, grid, s); break;
+        default: rc = launch_gax<GCT>(a, grid, s); break;   // kp % GCH > 112
+        }
+    }
+    return rc;
 }
 
 void ccx_hip_gbx_geometry(int nup, int kp, int *nchunk, int *ngg, int *gtr)

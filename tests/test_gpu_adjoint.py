@@ -43,9 +43,12 @@ def _close(got, want, rtol):
     assert err <= rtol, err
 
 
-@pytest.mark.parametrize("no,nu,seed", [(3, 5, 7), (4, 6, 11), (2, 17, 3), (5, 1, 2), (3, 33, 8), (4, 36, 1001)],
+@pytest.mark.parametrize("no,nu,seed", [(3, 5, 7), (4, 6, 11), (2, 17, 3), (5, 1, 2), (3, 33, 8), (4, 36, 1001),
+                                        (3, 113, 9)],
                          ids=lambda *a: None)
 def test_gradient_matches_dense_autograd(no, nu, seed):
+    """Padded blocks, nu=1, degenerate-only sets; K = nu + no padded to 4 covers back-contraction
+    K chunks of 1, 2 and 3 tiles (kp 8 .. 40) and, at nu=113 (kp=116), a chunk of 8 tiles."""
     inp = synth.make_inputs(no, nu, seed)
     want = dense.grad_dense(inp)
     got = _grad(inp)

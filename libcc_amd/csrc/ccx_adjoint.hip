@@ -466,13 +466,14 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
 
     // staging registers: Wbar block 4096 = 512 x 8, BX slice 16 x 128 = 512 x 4
     double rw[8], rb[4];
+    double rwgt = 1.0;                    // weight of the staged entry, applied when stashing
     auto fetch = [&](int step) {
         const int e = e_beg + step / a.nb, G = step % a.nb;
         const int tl = a.ent[2 * e], sg = a.ent[2 * e + 1];
         const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
         const int tpos[3] = {tp.x, tp.y, tp.z};
         const Sigma S = SIG[sg & 7];
-        const double wgt = (sg & 8) ? 2.0 : 1.0;   // a merged pair of a degenerate tuple
+        rwgt = (sg & 8) ? 2.0 : 1.0;      // a merged pair of a degenerate tuple
         int bc[3];
         bc[S.u] = R0; bc[S.v] = R1; bc[S.wf] = G;
         const double *wsrc = a.wbar + tl * cube + (((long long)bc[0] * a.nb + bc[1]) * a.nb + bc[2]) * 4096;
@@ -488,7 +489,7 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
             const int Kl = idx >> 4;
             double2 v = {0.0, 0.0};
             if (K0 + Kl < a.kp) v = reinterpret_cast<const double2 *>(bsrc + (long long)(K0 + Kl) * 16)[(idx & 15) >> 1];
-            rb[2 * k] = v.x * wgt; rb[2 * k + 1] = v.y * wgt;
+            rb[2 * k] = v.x; rb[2 * k + 1] = v.y;
         }
     };
     auto stash = [&](int buf) {
@@ -505,8 +506,8 @@ __global__ void __launch_bounds__(GTHR, 1) gax_kernel(GemmArgs a)
         for (int k = 0; k < 2; ++k) {
             const int idx = 2 * (tid + k * GTHR);
             const int Kl = idx >> 4, gl = idx & 15;
-            Bs[Kl * BS_STR + gl] = rb[2 * k];
-            Bs[Kl * BS_STR + gl + 1] = rb[2 * k + 1];
+            Bs[Kl * BS_STR + gl] = rb[2 * k] * rwgt;
+            Bs[Kl * BS_STR + gl + 1] = rb[2 * k + 1] * rwgt;
         }
     };
 
@@ -595,6 +596,7 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
     // along axis 2, so a wave reads 128-byte runs whatever sigma is.
     double2 rw[4];
     double ra[4];
+    double rwgt = 1.0;                    // weight of the staged entry, applied when stashing
     int st_tr;                            // 1: the slice's (hi, 2) axes are (o, g); 0: (g, o)
     auto fetch = [&](int step) {
         int r = step;
@@ -604,7 +606,7 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
         const int4 tp = reinterpret_cast<const int4 *>(a.tuples)[a.t0 + tl];
         const int tpos[3] = {tp.x, tp.y, tp.z};
         const Sigma S = SIG[sg & 7];
-        const double wgt = (sg & 8) ? 2.0 : 1.0;   // a merged pair of a degenerate tuple
+        rwgt = (sg & 8) ? 2.0 : 1.0;      // a merged pair of a degenerate tuple
         const int sa = (S.u == 0 || S.v == 0) ? 0 : 1;
         const int hi = sa == 0 ? 1 : 0;
         const int oa = (sa == S.u) ? S.v : S.u;
@@ -636,7 +638,7 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
         for (int k = 0; k < 4; ++k) {
             const int e = tid + k * GTHR;                 // (K, o) of the 128 x 16 slice
             const int Kl = e >> 4, o = e & 15, K = K0 + Kl;
-            ra[k] = K < a.kp ? src[((long long)(K >> 2) * 64 + (K & 3) * 16 + o) * 2] * wgt : 0.0;
+            ra[k] = K < a.kp ? src[((long long)(K >> 2) * 64 + (K & 3) * 16 + o) * 2] : 0.0;
         }
     };
     auto stash = [&](int buf) {
@@ -660,7 +662,7 @@ __global__ void __launch_bounds__(GTHR, 1) gbx_kernel(GemmArgs a)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int e = tid + k * GTHR;
-            As[(e >> 4) * BS_STR + (e & 15)] = ra[k];
+            As[(e >> 4) * BS_STR + (e & 15)] = ra[k] * rwgt;
         }
     };
 

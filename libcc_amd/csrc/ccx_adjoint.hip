@@ -201,73 +201,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
         g.e_part[item] = s * wgt;
     }
 
-    // pass 2: dVOE partials, dVOE(x_r1, x_r2, t[P1], t[P2]) += Z inv T1(t[P0], x_r0), summed
-    // over the item's block of the contracted axis r0
-#pragma unroll 1
-    for (int v = 0; v < 4; ++v) {
-        const int al = Mfma<double>::row(lane, v);
-        const int ag = A * TB + al;
-        double dB[2][6];      // b-axis contracted: summed over batches, then waves
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int p = 0; p < 6; ++p) dB[s][p] = 0.0;
-        static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
-            constexpr int bb = decltype(BB)::value;
-            const int bl = wave * BPW + bb;
-            const int bg = B * TB + bl;
-            double w[6];
-#pragma unroll
-            for (int p = 0; p < 6; ++p) w[p] = staged_w(ws, p, bb, v, tid);
-            const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
-            const int el[3] = {al, bl, cl};
-            static_for<6>([&](auto S) __attribute__((always_inline)) {
-                constexpr int s = decltype(S)::value;
-                constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
-                static_for<6>([&](auto PP) __attribute__((always_inline)) {
-                    constexpr int p = decltype(PP)::value;
-                    constexpr int q0 = wsrc_comp(p, s, 0), q1 = wsrc_comp(p, s, 2);
-                    constexpr int q2 = wsrc_comp(p, s, 1), q3 = wsrc_comp(p, s, 3);
-                    const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
-                    const double dv = z * inv * T[(PERM[p][0] * 3 + r0) * 16 + el[r0]];
-                    if constexpr (s < 2) {
-                        // a-axis: sum over the four row groups, then over v by the same lane
-                        // (s = 0 -> target (b, c); s = 1 -> (c, b))
-                        double r = dv;
-                        r += __shfl_xor(r, 16, 64);
-                        r += __shfl_xor(r, 32, 64);
-                        if (lane < 16) D[(s * 6 + p) * 256 + el[r1] * 16 + el[r2]] += r;
-                    } else if constexpr (s < 4) {
-                        dB[s - 2][p] += dv;
-                    } else {
-                        // c-axis: sum over the 16 lanes of equal row (s = 4 -> (a,b); 5 -> (b,a))
-                        double r = dv;
-                        r += __shfl_xor(r, 1, 64);
-                        r += __shfl_xor(r, 2, 64);
-                        r += __shfl_xor(r, 4, 64);
-                        r += __shfl_xor(r, 8, 64);
-                        if (cl == 0) D[(s * 6 + p) * 256 + el[r1] * 16 + el[r2]] = r;
-                    }
-                });
-            });
-        });
-        // b-axis: add the wave partials in wave order (deterministic)
-        for (int w2 = 0; w2 < NWAVE; ++w2) {
-            if (wave == w2) {
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int p = 0; p < 6; ++p) {
-                        // s = 2: (1,0,2) -> target (a, c); s = 3: (1,2,0) -> target (c, a)
-                        const int idx = s == 0 ? al * 16 + cl : cl * 16 + al;
-                        D[((s + 2) * 6 + p) * 256 + idx] += dB[s][p];
-                    }
-            }
-            __syncthreads();
-        }
-    }
-
-    // pass 3: Wbar at the six placements x^s = (x[PERM[s][0]], x[PERM[s][1]], x[PERM[s][2]])
+    // pass 2: Wbar at the six placements x^s = (x[PERM[s][0]], x[PERM[s][1]], x[PERM[s][2]])
 #pragma unroll 1
     for (int v = 0; v < 4; ++v) {
         const int al = Mfma<double>::row(lane, v);
@@ -319,6 +253,80 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
                 if (first) wb[(((long long)X * nb + Y) * nb + Z) * 4096 + (e0 * 16 + e1) * 16 + e2] = val;
             });
         });
+    }
+
+    __syncthreads();   // every wave is done with the staged VOE blocks
+
+    // pass 3 (after the Wbar pass, so that the staged VOE region is free for the b-axis
+    // partials): dVOE partials, dVOE(x_r1, x_r2, t[P1], t[P2]) += Z inv T1(t[P0], x_r0), summed
+    // over the item's block of the contracted axis r0
+#pragma unroll 1
+    for (int v = 0; v < 4; ++v) {
+        const int al = Mfma<double>::row(lane, v);
+        const int ag = A * TB + al;
+        double dB[2][6];      // b-axis contracted: summed over batches, then waves
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) dB[s][p] = 0.0;
+        static_for<BPW>([&](auto BB) __attribute__((always_inline)) {
+            constexpr int bb = decltype(BB)::value;
+            const int bl = wave * BPW + bb;
+            const int bg = B * TB + bl;
+            double w[6];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) w[p] = staged_w(ws, p, bb, v, tid);
+            const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
+            const int el[3] = {al, bl, cl};
+            static_for<6>([&](auto S) __attribute__((always_inline)) {
+                constexpr int s = decltype(S)::value;
+                constexpr int r0 = PERM[s][0], r1 = PERM[s][1], r2 = PERM[s][2];
+                static_for<6>([&](auto PP) __attribute__((always_inline)) {
+                    constexpr int p = decltype(PP)::value;
+                    constexpr int q0 = wsrc_comp(p, s, 0), q1 = wsrc_comp(p, s, 2);
+                    constexpr int q2 = wsrc_comp(p, s, 1), q3 = wsrc_comp(p, s, 3);
+                    const double z = 2.0 * w[q0] - 2.0 * w[q1] - w[q2] + w[q3];
+                    const double dv = z * inv * T[(PERM[p][0] * 3 + r0) * 16 + el[r0]];
+                    if constexpr (s < 2) {
+                        // a-axis: sum over the four row groups, then over v by the same lane
+                        // (s = 0 -> target (b, c); s = 1 -> (c, b))
+                        double r = dv;
+                        r += __shfl_xor(r, 16, 64);
+                        r += __shfl_xor(r, 32, 64);
+                        if (lane < 16) D[(s * 6 + p) * 256 + el[r1] * 16 + el[r2]] += r;
+                    } else if constexpr (s < 4) {
+                        dB[s - 2][p] += dv;
+                    } else {
+                        // c-axis: sum over the 16 lanes of equal row (s = 4 -> (a,b); 5 -> (b,a))
+                        double r = dv;
+                        r += __shfl_xor(r, 1, 64);
+                        r += __shfl_xor(r, 2, 64);
+                        r += __shfl_xor(r, 4, 64);
+                        r += __shfl_xor(r, 8, 64);
+                        if (cl == 0) D[(s * 6 + p) * 256 + el[r1] * 16 + el[r2]] = r;
+                    }
+                });
+            });
+        });
+        // b-axis: each wave's partials to its own slot of PB (the staged VOE region, no longer
+        // read), then the 12 x 64 targets of this row step summed over the waves in wave
+        // order (deterministic)
+        double *PB = V;   // [NWAVE][12][64]
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) PB[(wave * 12 + s * 6 + p) * 64 + lane] = dB[s][p];
+        __syncthreads();
+        for (int t = tid; t < 12 * 64; t += NTHR) {
+            const int ln = t & 63, k = t >> 6, s = k / 6, p = k % 6;
+            double sum = 0.0;
+            for (int w2 = 0; w2 < NWAVE; ++w2) sum += PB[(w2 * 12 + k) * 64 + ln];
+            // s = 2: (1,0,2) -> target (a, c); s = 3: (1,2,0) -> target (c, a)
+            const int a_l = Mfma<double>::row(ln, v), c_l = ln & 15;
+            const int idx = s == 0 ? a_l * 16 + c_l : c_l * 16 + a_l;
+            D[((s + 2) * 6 + p) * 256 + idx] += sum;
+        }
+        __syncthreads();
     }
 
     // ---- dVOE partials -> slots[tl][p][X][Y][Z]; placements that land on the same block

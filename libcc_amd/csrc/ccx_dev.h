@@ -26,7 +26,8 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
                     int ntup, int nblk, int grid, const void *const *split, int kps, double wscale, void *stream);
 int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, void *axr_s, void *axc_s, void *bx_s,
                        unsigned long long *dmax, int no, int nb, int kp, int kps, double *wscale, void *stream);
-int ccx_hip_to_f32(const double *src, float *dst, long long n, void *stream);
+int ccx_hip_pack_f32v(const double *axr, const double *axc, const double *bx, float *axr32, float *axc32,
+                      float *bx32, int no, int nb, int kp, int kpf, void *stream);
 int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tuples, const int *blocks,
                      const double *t1, double *v1t, double *v1, double *out, int no, int nu, int ntup, int nblk,
                      void *stream);

@@ -21,6 +21,7 @@ struct ccx_ctx {
     int loaded;
     double *d_axr, *d_axc, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
     float *d_axr32, *d_axc32, *d_bx32; /* FP32 form slabs, converted on first use after a load */
+    int64_t kpf;                        /* K padded to 32 for the FP32 form */
     int f32_ready;
     void *d_axr_s, *d_axc_s, *d_bx_s;  /* split BF16 form: 3 bf16 slices per element, built on first use */
     unsigned long long *d_max;
@@ -139,6 +140,7 @@ int ccx_create(int64_t no, int64_t nu, ccx_ctx **out)
     c->nb = c->nup / TB;
     c->kp = (nu + no + 3) / 4 * 4;
     c->kps = (nu + no + 63) / 64 * 64;
+    c->kpf = (nu + no + 31) / 32 * 32;
     c->ntup = build_tuples(no, NULL);
     c->nblk = build_blocks(c->nb, NULL);
     c->h_tuples = (int *)malloc(sizeof(int) * 4 * (size_t)(c->ntup > 0 ? c->ntup : 1));
@@ -304,7 +306,7 @@ static int grid_size(const ccx_ctx *c, int64_t nitems)
 static int ensure_f32(ccx_ctx *c)
 {
     if (c->f32_ready) return CCX_OK;
-    const int64_t nax = c->no * c->nb * c->nb * c->kp * 256, nbx = c->no * c->no * c->kp * c->nup;
+    const int64_t nax = c->no * c->nb * c->nb * 256 * c->kpf, nbx = c->no * c->no * c->nb * 16 * c->kpf;
     if (!c->d_axr32) {
         if (ccx_hip_malloc((void **)&c->d_axr32, sizeof(float) * (size_t)nax) != 0 ||
             ccx_hip_malloc((void **)&c->d_axc32, sizeof(float) * (size_t)nax) != 0 ||
@@ -314,9 +316,8 @@ static int ensure_f32(ccx_ctx *c)
             return CCX_ERR_ALLOC;
         }
     }
-    if (ccx_hip_to_f32(c->d_axr, c->d_axr32, nax, c->stream) != 0 ||
-        ccx_hip_to_f32(c->d_axc, c->d_axc32, nax, c->stream) != 0 ||
-        ccx_hip_to_f32(c->d_bx, c->d_bx32, nbx, c->stream) != 0)
+    if (ccx_hip_pack_f32v(c->d_axr, c->d_axc, c->d_bx, c->d_axr32, c->d_axc32, c->d_bx32, (int)c->no, (int)c->nb,
+                          (int)c->kp, (int)c->kpf, c->stream) != 0)
         return CCX_ERR_DEVICE;
     c->f32_ready = 1;
     return CCX_OK;
@@ -369,8 +370,8 @@ int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v
     if (nitems > 0 &&
         ccx_hip_triples(form, axr, axc, bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
                         c->d_v1part, NULL, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp,
-                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), split, (int)c->kps,
-                        c->split_scale, c->stream) != 0)
+                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), split,
+                        form == CCX_FORM_FP32 ? (int)c->kpf : (int)c->kps, c->split_scale, c->stream) != 0)
         return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[2], c->stream) != 0) return CCX_ERR_DEVICE;
     if (ccx_hip_finalize(c->d_epart, c->d_v1part, c->d_tuples, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,

@@ -607,6 +607,127 @@ __device__ __forceinline__ void item_w_split(const TriplesArgs &g, double *lds, 
     complete_orderings<MODE, float>(lds, acc, wave, lane);
 }
 
+// ---- FP32 K sweep -------------------------------------------------------------------
+typedef const __attribute__((address_space(1))) f4 gf4;   // global memory, 16 B per lane
+
+// FP32 form, blocks of 16 K: one 16-byte load per lane carries four v_mfma_f32_16x16x4_f32
+// k-steps of one batch row (lane l: inner index l & 15, float j at K = 16 kb + 4 j + (l >> 4)).
+// The BX fragments of the phase are staged in LDS (one 1 KB chunk per BX operand and block,
+// double buffered); the AX fragments stream per operand slot through a two-deep register
+// ring, as in the split sweep.  4 bytes per operand element instead of FP64's 8.
+template <int PHASE, int MODE>
+__device__ __forceinline__ void run_phase_f32v(const TriplesArgs &g, const ItemCtx &it, f4 (&acc)[6][BPW],
+                                               f4 *lbx, int wave, int lane)
+{
+    constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
+    constexpr int NSL = __builtin_popcount(SM), NBX = __builtin_popcount(XM);
+    static_assert(NBX <= NWAVE, "one BX chunk per wave");
+    const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
+    int r0[2], r1[2];
+    bool col[2];
+    if (PHASE == 0) { r0[0] = A; r1[0] = C; col[0] = true;  r0[1] = C; r1[1] = A; col[1] = false; }
+    else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
+    else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
+    const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
+    const int nkb = g.kps / 16;
+    // units of 16 B: an AX panel is 16 outer rows x nkb blocks x 64 lanes, a BX panel nkb x 64
+    gf4 *ap[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int o = s / 3, xp = s % 3;
+        const long long panel = ((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o];
+        ap[s] = (gf4 *)(col[o] ? g.axc : g.axr) + panel * (16LL * nkb * 64) + (long long)wave * (2 * nkb * 64);
+    }
+    gf4 *xq[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        xq[q] = (gf4 *)g.bx + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * ((long long)nkb * 64);
+
+    f4 br;
+    auto bx_load = [&](int kb) __attribute__((always_inline)) {
+        static_for<NBX>([&](auto Cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(Cc)::value;
+            if (wave == c) br = xq[nth_bit(XM, c)][kb * 64 + lane];
+        });
+    };
+    auto bx_store = [&](int buf) __attribute__((always_inline)) {
+        static_for<NBX>([&](auto Cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(Cc)::value;
+            if (wave == c) lbx[(buf * NBX + c) * 64 + lane] = br;
+        });
+    };
+    f4 ab[2][BPW];
+    auto a_load = [&](int s, int kb, f4 (&d)[BPW]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) d[bb] = ap[s][(bb * nkb + kb) * 64 + lane];
+    };
+    auto compute = [&](auto Sc, const f4 (&a)[BPW], int buf) __attribute__((always_inline)) {
+        constexpr int s = decltype(Sc)::value;
+        constexpr SlotTerms ST = slot_terms(PHASE, MODE, s);
+        f4 bs[ST.n > 0 ? ST.n : 1];
+        static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+            constexpr int n = decltype(N)::value;
+            constexpr int rank = __builtin_popcount(XM & ((1 << ST.b[n]) - 1));
+            bs[n] = lbx[(buf * NBX + rank) * 64 + lane];
+        });
+        static_for<4>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+                constexpr int n = decltype(N)::value;
+                constexpr int p = ST.p[n];
+#pragma unroll
+                for (int bb = 0; bb < BPW; ++bb) {
+                    if constexpr (PHASE == 2)
+                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bs[n][j], a[bb][j], acc[p][bb], 0, 0, 0);
+                    else
+                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[bb][j], bs[n][j], acc[p][bb], 0, 0, 0);
+                }
+            });
+        });
+    };
+
+    bx_load(0);
+    bx_store(0);
+    __syncthreads();
+    a_load(nth_bit(SM, 0), 0, ab[0]);
+    for (int kb = 0; kb < nkb; kb += 2) {
+        static_for<2>([&](auto HB) __attribute__((always_inline)) {
+            constexpr int hb = decltype(HB)::value;
+            const int kbb = kb + hb;
+            bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+            static_for<NSL>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                constexpr int t = hb * NSL + j;
+                constexpr int s = nth_bit(SM, j);
+                constexpr int sn = nth_bit(SM, (j + 1) % NSL);
+                const int kn = j + 1 < NSL ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+                a_load(sn, kn, ab[(t + 1) % 2]);
+                __builtin_amdgcn_sched_barrier(0);
+                compute(std::integral_constant<int, s>{}, ab[t % 2], hb);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            bx_store(1 - hb);
+            __syncthreads();
+        });
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void item_w_f32v(const TriplesArgs &g, double *lds, const ItemCtx &it,
+                                            f4 (&acc)[6][BPW], int wave, int lane)
+{
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int bb = 0; bb < BPW; ++bb) acc[p][bb] = f4{0, 0, 0, 0};
+    f4 *lbx = reinterpret_cast<f4 *>(lds);
+    run_phase_f32v<0, MODE>(g, it, acc, lbx, wave, lane);
+    transpose_lb_to_l<MODE, float>(lds, acc, wave, lane);
+    run_phase_f32v<1, MODE>(g, it, acc, lbx, wave, lane);
+    run_phase_f32v<2, MODE>(g, it, acc, lbx, wave, lane);
+    complete_orderings<MODE, float>(lds, acc, wave, lane);
+}
+
 
 }  // namespace
 

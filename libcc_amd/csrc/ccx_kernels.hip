@@ -206,12 +206,14 @@ __global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__rest
 // ---------------------------------------------------------------------------------
 namespace {
 
-template <int MODE, class T, bool SPLIT>
+// FK: 0 FP64 slabs, 1 FP32 slabs (16 K per fragment load), 2 split BF16 slices
+template <int MODE, class T, int FK>
 __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
                         int tkind, int mult, int wave, int lane, int tid)
 {
     typename Mfma<T>::acc_t acc[6][BPW];
-    if constexpr (SPLIT) item_w_split<MODE>(g, lds, it, acc, wave, lane);
+    if constexpr (FK == 2) item_w_split<MODE>(g, lds, it, acc, wave, lane);
+    else if constexpr (FK == 1) item_w_f32v<MODE>(g, lds, it, acc, wave, lane);
     else item_w<MODE, T>(g, lds, it, acc, wave, lane);
 
     const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
@@ -257,7 +259,7 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
 #pragma unroll
             for (int p = 0; p < 6; ++p) {
                 w[p] = (double)acc[p][bb][v];
-                if constexpr (SPLIT) w[p] *= g.wscale;
+                if constexpr (FK == 2) w[p] *= g.wscale;
             }
             const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
             const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
@@ -348,7 +350,7 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
 
 }  // namespace
 
-template <class T, bool SPLIT>
+template <class T, int FK>
 __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
 {
     extern __shared__ double lds[];
@@ -376,12 +378,12 @@ __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
         const long long item = (long long)t * g.nblk + q;   // storage index
         int tkind, mult;
         const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
-        if (tkind == 2) do_item<M_IJJ, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (tkind == 3) do_item<M_IIJ, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (mult == 1) do_item<M_FULL, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (mult == 6) do_item<M_AAA, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else do_item<M_ACC, T, SPLIT>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        if (tkind == 2) do_item<M_IJJ, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (tkind == 3) do_item<M_IIJ, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (mult == 1) do_item<M_FULL, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (mult == 6) do_item<M_AAA, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        else do_item<M_ACC, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
     }
 }
 
@@ -519,10 +521,35 @@ __global__ void __launch_bounds__(256) t1wt3_kernel(const double *__restrict__ w
 
 __global__ void zero_counter_kernel(unsigned int *c) { *c = 0u; }
 
-__global__ void to_f32_kernel(const double *__restrict__ src, float *__restrict__ dst, long long n)
+// FP32 slabs from the FP64 ones in the FP32 sweep's fragment order: AX [panel][outer][K block
+// of 16][lane][4], BX [panel][K block][lane][4]; lane l holds inner index l & 15, element j
+// is at K = 16 kb + 4 j + (l >> 4).  BX = 1: the source is the BX slab [panel][K][16 gamma].
+template <int BX>
+__global__ void pack_f32v_kernel(const double *__restrict__ src, float *__restrict__ dst, long long npanel, int kp,
+                                 int kpf)
 {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        dst[i] = (float)src[i];
+    const int nkb = kpf / 16;
+    const long long total = npanel * (BX ? 16LL : 256LL) * kpf;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(idx & 3), lane = (int)((idx >> 2) & 63);
+        const long long rr = idx >> 8;     // (panel, outer, kb) or (panel, kb)
+        const int kb = (int)(rr % nkb);
+        const long long po = rr / nkb;
+        const int K = kb * 16 + j * 4 + (lane >> 4), inner = lane & 15;
+        double v = 0.0;
+        if (K < kp) {
+            if (BX) {
+                v = src[po * (16LL * kp) + (long long)K * 16 + inner];
+            } else {
+                const int outer = (int)(po % 16);
+                const long long panel = po / 16;
+                v = src[panel * (256LL * kp) + (((long long)(outer >> 1) * (kp / 4) + K / 4) * 64 + (K & 3) * 16 + inner) * 2 +
+                        (outer & 1)];
+            }
+        }
+        dst[idx] = (float)v;
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -601,7 +628,8 @@ int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *v
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// form 0: FP64 slabs; 1: FP32 slabs; 2: split BF16 (split[] = axr_s, axc_s, bx_s, W scaled by wscale)
+// form 0: FP64 slabs; 1: FP32 slabs in the FP32 sweep's order (K padded to kps, a multiple of
+// 32); 2: split BF16 (split[] = axr_s, axc_s, bx_s, K padded to kps, W scaled by wscale)
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
@@ -611,9 +639,10 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     const size_t lds_bytes = sizeof(double) * LDS_DOUBLES;
     if (form < 0 || form > 2) return 3;
     if (form == 2 && (split == nullptr || kps % 64 != 0)) return 3;
+    if (form == 1 && kps % 32 != 0) return 3;   // FP32 slabs: K padded to 32 (kps)
     if (!attr_set[form]) {
-        const void *fn = form == 0 ? (const void *)triples_kernel<double, false>
-                       : form == 1 ? (const void *)triples_kernel<float, false> : (const void *)triples_kernel<float, true>;
+        const void *fn = form == 0 ? (const void *)triples_kernel<double, 0>
+                       : form == 1 ? (const void *)triples_kernel<float, 1> : (const void *)triples_kernel<float, 2>;
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return 2;
         attr_set[form] = true;
@@ -637,9 +666,9 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     if (grid > g.nitems) grid = (int)g.nitems;
     hipStream_t s = (hipStream_t)stream;
     zero_counter_kernel<<<1, 1, 0, s>>>(counter);
-    if (form == 0) triples_kernel<double, false><<<grid, NTHR, lds_bytes, s>>>(g);
-    else if (form == 1) triples_kernel<float, false><<<grid, NTHR, lds_bytes, s>>>(g);
-    else triples_kernel<float, true><<<grid, NTHR, lds_bytes, s>>>(g);
+    if (form == 0) triples_kernel<double, 0><<<grid, NTHR, lds_bytes, s>>>(g);
+    else if (form == 1) triples_kernel<float, 1><<<grid, NTHR, lds_bytes, s>>>(g);
+    else triples_kernel<float, 2><<<grid, NTHR, lds_bytes, s>>>(g);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -699,9 +728,15 @@ int ccx_hip_t1wt3(const double *w, const double *voe6, double dijk, const double
 
 int ccx_hip_lds_bytes(void) { return (int)(sizeof(double) * LDS_DOUBLES); }
 
-int ccx_hip_to_f32(const double *src, float *dst, long long n, void *stream)
+int ccx_hip_pack_f32v(const double *axr, const double *axc, const double *bx, float *axr32, float *axc32,
+                      float *bx32, int no, int nb, int kp, int kpf, void *stream)
 {
-    to_f32_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(src, dst, n);
+    if (kpf % 32 != 0) return 3;
+    hipStream_t s = (hipStream_t)stream;
+    const long long nax = (long long)no * nb * nb * 256 * kpf, nbx = (long long)no * no * nb * 16 * kpf;
+    pack_f32v_kernel<0><<<grid_for(nax), 256, 0, s>>>(axr, axr32, (long long)no * nb * nb, kp, kpf);
+    pack_f32v_kernel<0><<<grid_for(nax), 256, 0, s>>>(axc, axc32, (long long)no * nb * nb, kp, kpf);
+    pack_f32v_kernel<1><<<grid_for(nbx), 256, 0, s>>>(bx, bx32, (long long)no * no * nb, kp, kpf);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

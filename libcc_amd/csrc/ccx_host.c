@@ -21,7 +21,7 @@ struct ccx_ctx {
     int loaded;
     double *d_axr, *d_axc, *d_bx, *d_voep, *d_eh, *d_ep, *d_t1;
     float *d_axr32, *d_axc32, *d_bx32; /* FP32 form slabs, converted on first use after a load */
-    int64_t kpf;                        /* K padded to 32 for the FP32 form */
+    int64_t kpf;                        /* K padded to 48 for the FP32 form */
     int f32_ready;
     void *d_axr_s, *d_axc_s, *d_bx_s;  /* split BF16 form: 3 bf16 slices per element, built on first use */
     unsigned long long *d_max;
@@ -140,7 +140,7 @@ int ccx_create(int64_t no, int64_t nu, ccx_ctx **out)
     c->nb = c->nup / TB;
     c->kp = (nu + no + 3) / 4 * 4;
     c->kps = (nu + no + 63) / 64 * 64;
-    c->kpf = (nu + no + 31) / 32 * 32;
+    c->kpf = (nu + no + 47) / 48 * 48;
     c->ntup = build_tuples(no, NULL);
     c->nblk = build_blocks(c->nb, NULL);
     c->h_tuples = (int *)malloc(sizeof(int) * 4 * (size_t)(c->ntup > 0 ? c->ntup : 1));

@@ -613,8 +613,9 @@ typedef const __attribute__((address_space(1))) f4 gf4;   // global memory, 16 B
 // FP32 form, blocks of 16 K: one 16-byte load per lane carries four v_mfma_f32_16x16x4_f32
 // k-steps of one batch row (lane l: inner index l & 15, float j at K = 16 kb + 4 j + (l >> 4)).
 // The BX fragments of the phase are staged in LDS (one 1 KB chunk per BX operand and block,
-// double buffered); the AX fragments stream per operand slot through a two-deep register
-// ring, as in the split sweep.  4 bytes per operand element instead of FP64's 8.
+// three buffers); the AX fragments stream per operand slot through a three-deep register ring
+// that runs two slot steps ahead.  Three blocks per loop iteration (K padded to 48) keep every
+// ring and buffer index a compile-time constant.  4 bytes per operand element instead of 8.
 template <int PHASE, int MODE>
 __device__ __forceinline__ void run_phase_f32v(const TriplesArgs &g, const ItemCtx &it, f4 (&acc)[6][BPW],
                                                f4 *lbx, int wave, int lane)
@@ -656,7 +657,7 @@ __device__ __forceinline__ void run_phase_f32v(const TriplesArgs &g, const ItemC
             if (wave == c) lbx[(buf * NBX + c) * 64 + lane] = br;
         });
     };
-    f4 ab[2][BPW];
+    f4 ab[3][BPW];
     auto a_load = [&](int s, int kb, f4 (&d)[BPW]) __attribute__((always_inline)) {
 #pragma unroll
         for (int bb = 0; bb < BPW; ++bb) d[bb] = ap[s][(bb * nkb + kb) * 64 + lane];
@@ -689,9 +690,11 @@ __device__ __forceinline__ void run_phase_f32v(const TriplesArgs &g, const ItemC
     bx_load(0);
     bx_store(0);
     __syncthreads();
+    // steps t = block * NSL + slot index; the ring holds steps t, t+1, t+2
     a_load(nth_bit(SM, 0), 0, ab[0]);
-    for (int kb = 0; kb < nkb; kb += 2) {
-        static_for<2>([&](auto HB) __attribute__((always_inline)) {
+    a_load(nth_bit(SM, 1 % NSL), 1 / NSL < nkb ? 1 / NSL : nkb - 1, ab[1]);
+    for (int kb = 0; kb < nkb; kb += 3) {
+        static_for<3>([&](auto HB) __attribute__((always_inline)) {
             constexpr int hb = decltype(HB)::value;
             const int kbb = kb + hb;
             bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1);
@@ -699,14 +702,15 @@ __device__ __forceinline__ void run_phase_f32v(const TriplesArgs &g, const ItemC
                 constexpr int j = decltype(J)::value;
                 constexpr int t = hb * NSL + j;
                 constexpr int s = nth_bit(SM, j);
-                constexpr int sn = nth_bit(SM, (j + 1) % NSL);
-                const int kn = j + 1 < NSL ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
-                a_load(sn, kn, ab[(t + 1) % 2]);
+                constexpr int t2 = t + 2;                       // the step loaded now
+                constexpr int s2 = nth_bit(SM, t2 % NSL);
+                const int k2 = kb + t2 / NSL;
+                a_load(s2, k2 < nkb ? k2 : nkb - 1, ab[t2 % 3]);
                 __builtin_amdgcn_sched_barrier(0);
-                compute(std::integral_constant<int, s>{}, ab[t % 2], hb);
+                compute(std::integral_constant<int, s>{}, ab[t % 3], hb);
                 __builtin_amdgcn_sched_barrier(0);
             });
-            bx_store(1 - hb);
+            bx_store((hb + 1) % 3);
             __syncthreads();
         });
     }

@@ -629,7 +629,7 @@ int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *v
 }
 
 // form 0: FP64 slabs; 1: FP32 slabs in the FP32 sweep's order (K padded to kps, a multiple of
-// 32); 2: split BF16 (split[] = axr_s, axc_s, bx_s, K padded to kps, W scaled by wscale)
+// 48); 2: split BF16 (split[] = axr_s, axc_s, bx_s, K padded to kps, W scaled by wscale)
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
@@ -639,7 +639,7 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     const size_t lds_bytes = sizeof(double) * LDS_DOUBLES;
     if (form < 0 || form > 2) return 3;
     if (form == 2 && (split == nullptr || kps % 64 != 0)) return 3;
-    if (form == 1 && kps % 32 != 0) return 3;   // FP32 slabs: K padded to 32 (kps)
+    if (form == 1 && kps % 48 != 0) return 3;   // FP32 slabs: K padded to 48 (kps)
     if (!attr_set[form]) {
         const void *fn = form == 0 ? (const void *)triples_kernel<double, 0>
                        : form == 1 ? (const void *)triples_kernel<float, 1> : (const void *)triples_kernel<float, 2>;
@@ -731,7 +731,7 @@ int ccx_hip_lds_bytes(void) { return (int)(sizeof(double) * LDS_DOUBLES); }
 int ccx_hip_pack_f32v(const double *axr, const double *axc, const double *bx, float *axr32, float *axc32,
                       float *bx32, int no, int nb, int kp, int kpf, void *stream)
 {
-    if (kpf % 32 != 0) return 3;
+    if (kpf % 48 != 0) return 3;
     hipStream_t s = (hipStream_t)stream;
     const long long nax = (long long)no * nb * nb * 256 * kpf, nbx = (long long)no * no * nb * 16 * kpf;
     pack_f32v_kernel<0><<<grid_for(nax), 256, 0, s>>>(axr, axr32, (long long)no * nb * nb, kp, kpf);

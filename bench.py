@@ -107,12 +107,12 @@ def cpu_baseline(inp_dev: dict, no: int, nu: int, budget_s: float, threads: int)
             "tuples_per_s": 1.0 / per}
 
 
-def split_adjoint(ctx, g64: dict) -> dict:
-    """The adjoint with W recomputed in the split BF16x3 form: time and, per gradient array,
-    the error against the FP64 adjoint in the relative Frobenius norm."""
+def reduced_adjoint(ctx, g64: dict, form: str) -> dict:
+    """The adjoint with W recomputed in a reduced form (fp32, bf16x3): time and, per gradient
+    array, the error against the FP64 adjoint in the relative Frobenius norm."""
     import torch
 
-    g = ctx.gradient(device=True, form="bf16x3")
+    g = ctx.gradient(device=True, form=form)
     ms = ctx.timing()[0]
     err = {k: float(torch.linalg.norm(g[k] - g64[k]) / torch.linalg.norm(g64[k]))
            for k in ("t1", "t2", "vm", "voe", "vvvo")}
@@ -133,8 +133,8 @@ def main() -> None:
     ap.add_argument("--rocpd-summary", nargs="+", default=None, metavar="DB",
                     help="only summarize rocprofv3 result databases as markdown")
     ap.add_argument("--adjoint-only", action="store_true",
-                    help="only run one adjoint (gradient) evaluation and print its timing (with --form bf16x3 "
-                         "also the split-form adjoint and its gradient error against FP64)")
+                    help="only run one adjoint (gradient) evaluation and print its timing (with --form fp32 or "
+                         "bf16x3 also that form's adjoint and its gradient error against FP64)")
     ap.add_argument("--form", default="fp64", choices=sorted(FORM_PEAK),
                     help="arithmetic form of the timed steps (the headline metric is fp64)")
     ap.add_argument("--no-extra", action="store_true",
@@ -175,10 +175,10 @@ def main() -> None:
         g = ctx.gradient(device=True)
         out = {"adjoint_ms": ctx.timing()[0], "e_t": g["e"], "no": no, "nu": nu}
         log(f"adjoint fp64 done: {out['adjoint_ms']:.0f} ms")
-        if args.form == "bf16x3":
-            log("adjoint bf16x3")
-            out["bf16x3"] = split_adjoint(ctx, g)
-            log(f"adjoint bf16x3 done: {out['bf16x3']['ms']:.0f} ms")
+        if args.form != "fp64":
+            log(f"adjoint {args.form}")
+            out[args.form] = reduced_adjoint(ctx, g, args.form)
+            log(f"adjoint {args.form} done: {out[args.form]['ms']:.0f} ms")
         print(json.dumps(out), flush=True)
         ctx.close()
         return
@@ -255,10 +255,11 @@ def main() -> None:
         out["adjoint"] = {"ms": adj_ms, "ratio_to_forward": adj_ms / (kernel_s * 1e3),
                           "flops": 3.0 * flops,
                           "frac_of_fp64_peak": 3.0 * flops / (adj_ms / 1e3) / 1e12 / PEAK_FP64_TFLOPS}
-        log("adjoint bf16x3")
-        sa = split_adjoint(ctx, g)
-        sa["ratio_to_forward"] = sa["ms"] / (kernel_s * 1e3)
-        out["adjoint"]["bf16x3"] = sa
+        for form in ("fp32", "bf16x3"):
+            log(f"adjoint {form}")
+            sa = reduced_adjoint(ctx, g, form)
+            sa["ratio_to_forward"] = sa["ms"] / (kernel_s * 1e3)
+            out["adjoint"][form] = sa
         del g
     ctx.close()
     if not args.no_cpu:

@@ -99,9 +99,10 @@ int ccx_gradient(ccx_ctx *ctx, double *ets, double *etd, double *dt1, double *dt
                  double *dvvvo, int on_device);
 
 /* ccx_gradient with the arithmetic form of the per-tuple W recomputation: CCX_FORM_FP64 (as
- * ccx_gradient) or CCX_FORM_BF16X3 (W from the split BF16 slices with FP32 accumulation, then
- * FP64: dE/dW, every back-contraction and every gradient accumulation stay FP64).  Other
- * forms return CCX_ERR_FORM. */
+ * ccx_gradient), CCX_FORM_FP32 (W from FP32 operands) or CCX_FORM_BF16X3 (W from the split
+ * BF16 slices); both reduced forms accumulate W in FP32, and after it everything is FP64:
+ * dE/dW, every back-contraction and every gradient accumulation.  Other forms return
+ * CCX_ERR_FORM. */
 int ccx_gradient_form(ccx_ctx *ctx, int form, double *ets, double *etd, double *dt1, double *dt2, double *dvm,
                       double *dvoe, double *dvvvo, int on_device);
 

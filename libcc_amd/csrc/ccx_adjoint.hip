@@ -352,10 +352,10 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
 
 }  // namespace
 
-// SPLIT: W is recomputed by the split BF16x3 K sweep (FP32 accumulation, layout row
-// 4 (lane >> 4) + v) and staged, scaled to FP64, in the FP64 layout (row (lane >> 4) + 4 v) that
-// the epilogue reads; everything after the staging is the FP64 path.
-template <bool SPLIT>
+// FK 1 / 2: W is recomputed by the FP32 or the split BF16x3 K sweep (FP32 accumulation,
+// layout row 4 (lane >> 4) + v) and staged, as FP64, in the FP64 layout (row (lane >> 4) + 4 v)
+// that the epilogue reads; everything after the staging is the FP64 path.  FK 0: FP64 sweep.
+template <int FK>
 __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
 {
     extern __shared__ double lds[];
@@ -378,14 +378,23 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
         int tkind, mult;
         const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
         double *ws = ga.wstage + (long long)blockIdx.x * (6 * BPW * 4 * NTHR);
-        if constexpr (SPLIT) {
+        if constexpr (FK != 0) {
             f4 acc[6][BPW];
-            if (tkind == 2) item_w_split<M_IJJ>(g, lds, it, acc, wave, lane);
-            else if (tkind == 3) item_w_split<M_IIJ>(g, lds, it, acc, wave, lane);
-            else if (mult == 1) item_w_split<M_FULL>(g, lds, it, acc, wave, lane);
-            else if (mult == 6) item_w_split<M_AAA>(g, lds, it, acc, wave, lane);
-            else if (it.blk[0] == it.blk[1]) item_w_split<M_AAC>(g, lds, it, acc, wave, lane);
-            else item_w_split<M_ACC>(g, lds, it, acc, wave, lane);
+            if constexpr (FK == 2) {
+                if (tkind == 2) item_w_split<M_IJJ>(g, lds, it, acc, wave, lane);
+                else if (tkind == 3) item_w_split<M_IIJ>(g, lds, it, acc, wave, lane);
+                else if (mult == 1) item_w_split<M_FULL>(g, lds, it, acc, wave, lane);
+                else if (mult == 6) item_w_split<M_AAA>(g, lds, it, acc, wave, lane);
+                else if (it.blk[0] == it.blk[1]) item_w_split<M_AAC>(g, lds, it, acc, wave, lane);
+                else item_w_split<M_ACC>(g, lds, it, acc, wave, lane);
+            } else {
+                if (tkind == 2) item_w_f32v<M_IJJ>(g, lds, it, acc, wave, lane);
+                else if (tkind == 3) item_w_f32v<M_IIJ>(g, lds, it, acc, wave, lane);
+                else if (mult == 1) item_w_f32v<M_FULL>(g, lds, it, acc, wave, lane);
+                else if (mult == 6) item_w_f32v<M_AAA>(g, lds, it, acc, wave, lane);
+                else if (it.blk[0] == it.blk[1]) item_w_f32v<M_AAC>(g, lds, it, acc, wave, lane);
+                else item_w_f32v<M_ACC>(g, lds, it, acc, wave, lane);
+            }
             // element (row 4 (lane >> 4) + v, column lane & 15) belongs to FP64-layout lane
             // 16 v + (lane & 15), slot lane >> 4 of the same wave
             const int dst = wave * 64 + (lane & 15);
@@ -858,18 +867,23 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
                       const double *ep, const double *t1, const int *tuples, const int *blocks, double *e_part,
                       double *v1_part, double *wbar, double *slots, double *wstage, unsigned int *counter, int no,
                       int nu, int nup, int kp, int ntup, int nblk, int t0, int tcount, int grid,
-                      const void *const *split, int kps, double wscale, void *stream)
+                      int form, const void *const *w32, int kps, double wscale, void *stream)
 {
-    static bool attr_set[2] = {false, false};
+    // form 0: FP64 slabs axr/axc/bx; 1: FP32 slabs w32[] (K padded to kps, a multiple of 48);
+    // 2: split BF16 slices w32[] (K padded to kps, a multiple of 64, W scaled by wscale)
+    static bool attr_set[3] = {false, false, false};
     const size_t lds_bytes = sizeof(double) * ADJ_LDS;
-    const int sp = split != nullptr;
-    if (sp && kps % 64 != 0) return 3;
-    if (!attr_set[sp]) {
-        const void *fn = sp ? (const void *)adj_item_kernel<true> : (const void *)adj_item_kernel<false>;
+    if (form < 0 || form > 2) return 3;
+    if (form != 0 && w32 == nullptr) return 3;
+    if ((form == 1 && kps % 48 != 0) || (form == 2 && kps % 64 != 0)) return 3;
+    if (!attr_set[form]) {
+        const void *fn = form == 0 ? (const void *)adj_item_kernel<0>
+                       : form == 1 ? (const void *)adj_item_kernel<1> : (const void *)adj_item_kernel<2>;
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return 2;
-        attr_set[sp] = true;
+        attr_set[form] = true;
     }
+    const int sp = form == 2;
     AdjArgs ga;
     TriplesArgs &g = ga.f;
     g.axr = axr; g.axc = axc; g.bx = bx; g.voe = voep; g.eh = eh; g.ep = ep; g.tuples = tuples;
@@ -877,9 +891,10 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
     g.block_major = 0;
-    g.axr_s = sp ? split[0] : nullptr;
-    g.axc_s = sp ? split[1] : nullptr;
-    g.bx_s = sp ? split[2] : nullptr;
+    if (form == 1) { g.axr = w32[0]; g.axc = w32[1]; g.bx = w32[2]; }
+    g.axr_s = sp ? w32[0] : nullptr;
+    g.axc_s = sp ? w32[1] : nullptr;
+    g.bx_s = sp ? w32[2] : nullptr;
     g.kps = kps;
     g.wscale = sp ? wscale : 1.0;
     ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.wstage = wstage; ga.t0 = t0; ga.tcount = tcount;
@@ -889,8 +904,9 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     if (grid > nlocal) grid = (int)nlocal;
     hipStream_t s = (hipStream_t)stream;
     if (hipMemsetAsync(counter, 0, sizeof(unsigned int), s) != hipSuccess) return 1;
-    if (sp) adj_item_kernel<true><<<grid, NTHR, lds_bytes, s>>>(ga);
-    else adj_item_kernel<false><<<grid, NTHR, lds_bytes, s>>>(ga);
+    if (form == 2) adj_item_kernel<2><<<grid, NTHR, lds_bytes, s>>>(ga);
+    else if (form == 1) adj_item_kernel<1><<<grid, NTHR, lds_bytes, s>>>(ga);
+    else adj_item_kernel<0><<<grid, NTHR, lds_bytes, s>>>(ga);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

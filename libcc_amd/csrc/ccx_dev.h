@@ -41,7 +41,7 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
                       const double *ep, const double *t1, const int *tuples, const int *blocks, double *e_part,
                       double *v1_part, double *wbar, double *slots, double *wstage, unsigned int *counter, int no,
                       int nu, int nup, int kp, int ntup, int nblk, int t0, int tcount, int grid,
-                      const void *const *split, int kps, double wscale, void *stream);
+                      int form, const void *const *w32, int kps, double wscale, void *stream);
 size_t ccx_hip_adj_stage_doubles(void);
 int ccx_hip_gax(const double *wbar, const double *bx, double *gax, const int *tuples, int t0, const int *xlist,
                 const int *xoff, const int *ent, int nx, int no, int nup, int kp, void *stream);

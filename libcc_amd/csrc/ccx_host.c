@@ -503,12 +503,18 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
 {
     if (!c) return CCX_ERR_ARG;
     if (!c->loaded) return CCX_ERR_STATE;
-    if (form != CCX_FORM_FP64 && form != CCX_FORM_BF16X3) return CCX_ERR_FORM;
+    if (form != CCX_FORM_FP64 && form != CCX_FORM_FP32 && form != CCX_FORM_BF16X3) return CCX_ERR_FORM;
     if (form == CCX_FORM_BF16X3) {
         int rs = ensure_split(c);
         if (rs != CCX_OK) return rs;
     }
-    const void *split[3] = {c->d_axr_s, c->d_axc_s, c->d_bx_s};
+    if (form == CCX_FORM_FP32) {
+        int rs = ensure_f32(c);
+        if (rs != CCX_OK) return rs;
+    }
+    const void *w32[3] = {form == CCX_FORM_FP32 ? (const void *)c->d_axr32 : c->d_axr_s,
+                          form == CCX_FORM_FP32 ? (const void *)c->d_axc32 : c->d_axc_s,
+                          form == CCX_FORM_FP32 ? (const void *)c->d_bx32 : c->d_bx_s};
     const int64_t no = c->no, nu = c->nu, nup = c->nup, nb = c->nb, kp = c->kp;
     const int64_t ntup = c->ntup, nblk = c->nblk;
     int rc = CCX_OK;
@@ -605,8 +611,8 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
                               c->d_blocks, c->d_epart, c->d_v1part, d_wbar, d_slots, d_wstage, c->d_counter, (int)no,
                               (int)nu,
                               (int)nup, (int)kp, (int)ntup, (int)nblk, (int)t0, tcount,
-                              grid_size(c, (int64_t)tcount * nblk), form == CCX_FORM_BF16X3 ? split : NULL,
-                              (int)c->kps, c->split_scale, c->stream) != 0 ||
+                              grid_size(c, (int64_t)tcount * nblk), form, form == CCX_FORM_FP64 ? NULL : w32,
+                              form == CCX_FORM_FP32 ? (int)c->kpf : (int)c->kps, c->split_scale, c->stream) != 0 ||
             ccx_hip_gax(d_wbar, c->d_bx, d_gax, c->d_tuples, (int)t0, dl[0][0], dl[0][1], dl[0][2], nkeys[0], (int)no,
                         (int)nup, (int)kp, c->stream) != 0 ||
             ccx_hip_gbx(d_wbar, c->d_axr, c->d_axc, d_gbx, d_gpart, c->d_tuples, (int)t0, dl[1][0], dl[1][1],

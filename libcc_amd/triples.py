@@ -84,7 +84,7 @@ class TriplesContext:
 
     def gradient(self, device: bool = False, form: str = "fp64") -> dict:
         """E(T) and dE/d(t1, t2, vm, voe, vvvo) in the reference layouts (ccx_gradient_form; form
-        "fp64" or "bf16x3" for the W recomputation, everything after it FP64).
+        "fp64", "fp32" or "bf16x3" for the W recomputation, everything after it FP64).
 
         With ``device=True`` the gradients are torch float64 tensors on the current GPU,
         written in place by the library; otherwise Fortran-ordered numpy arrays."""

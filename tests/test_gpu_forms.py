@@ -85,22 +85,27 @@ def test_unknown_form_rejected():
     ctx.close()
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("no,nu,seed", [(4, 36, 1001), (3, 17, 6), (8, 120, 1003), (2, 33, 4)], ids=lambda v: str(v))
-def test_split_adjoint_matches_fp64(no, nu, seed):
-    """ccx_gradient_form(CCX_FORM_BF16X3): W recomputed from the split slices, dE/dW and every
-    back-contraction in FP64.  Each gradient array within 1e-6 of the FP64 gradient in the
-    relative Frobenius norm (the bar the FP64 adjoint meets against finite differences)."""
+def test_reduced_adjoint_matches_fp64(no, nu, seed, form):
+    """ccx_gradient_form(CCX_FORM_FP32 / CCX_FORM_BF16X3): W recomputed in the reduced form,
+    dE/dW and every back-contraction in FP64.  Each gradient array within 1e-6 of the FP64
+    gradient in the relative Frobenius norm (the bar the FP64 adjoint meets against finite
+    differences)."""
     ctx = TriplesContext(no, nu)
     ctx.load(synth.make_inputs(no, nu, seed))
     want = ctx.gradient()
-    got = ctx.gradient(form="bf16x3")
+    got = ctx.gradient(form=form)
     assert abs(got["e"] - want["e"]) <= 1e-6 * abs(want["e"])
     for k in ("t1", "t2", "vm", "voe", "vvvo"):
         ref = np.linalg.norm(want[k])
         assert np.linalg.norm(got[k] - want[k]) <= 1e-6 * ref, k
-    again = ctx.gradient(form="bf16x3")
+    # the reduced W is the one the energy entry computes
+    e = ctx.energy(form)
+    assert abs(got["e"] - e["e"]) <= 1e-12 * abs(e["e"])
+    again = ctx.gradient(form=form)
     assert again["e"] == got["e"] and np.array_equal(again["vvvo"], got["vvvo"])
     ets, etd = ctypes.c_double(), ctypes.c_double()
-    assert lib().ccx_gradient_form(ctx._ctx, 1, ctypes.byref(ets), ctypes.byref(etd), None, None, None, None, None,
+    assert lib().ccx_gradient_form(ctx._ctx, 7, ctypes.byref(ets), ctypes.byref(etd), None, None, None, None, None,
                                    0) == 6
     ctx.close()

@@ -80,6 +80,20 @@ int ccx_energy(ccx_ctx *ctx, int form, double *ets, double *etd, double *v1, int
  * off the diagonal). */
 int ccx_tuple_w(ccx_ctx *ctx, int64_t i, int64_t j, int64_t k, double *w);
 
+/* Virtual-triple view of W: v3(i,j,k) = W_ijk(a,b,c) for one virtual triple (1-based a,b,c)
+ * over all occupied i,j,k (no^3, column-major, host), with v3(i,i,i) = 0.  This is the cube
+ * the triples-abc prototype abc_tuple_formv3(a,b,c,...,v3) assembles from 12 GEMMs and
+ * zerot3 (src/triples-abc/abc_tuple.f90:1-81).  That routine itself is not exported: it is
+ * never called, its array arguments are declared with sizes its GEMMs do not use, and no
+ * caller defines what its slices hold. */
+int ccx_w_abc(ccx_ctx *ctx, int64_t a, int64_t b, int64_t c, double *v3);
+
+/* E[T] (etd) re-summed over virtual triples: sum over a>b>c of X_abc plus half the sum of
+ * X_abb and X_aab, X_abc = sum_ijk W [8W - 4(W_ikj + W_kji + W_jik) + 2(W_kij + W_jki)] / D
+ * on the cubes of ccx_w_abc.  Equal to the etd of ccx_energy up to summation order; a
+ * cross-check route (FP64 FMA, not the matrix cores), not the fast path. */
+int ccx_energy_abc(ccx_ctx *ctx, double *etd);
+
 /* Timing of the last ccx_energy call in milliseconds (HIP events around the
  * persistent kernel only, and around the whole call). */
 int ccx_last_timing(const ccx_ctx *ctx, double *kernel_ms, double *total_ms);

@@ -54,6 +54,10 @@ def lib() -> ctypes.CDLL:
     L.ccx_energy.restype = ctypes.c_int
     L.ccx_tuple_w.argtypes = [ctxp, i64, i64, i64, dp]
     L.ccx_tuple_w.restype = ctypes.c_int
+    L.ccx_w_abc.argtypes = [ctxp, i64, i64, i64, dp]
+    L.ccx_w_abc.restype = ctypes.c_int
+    L.ccx_energy_abc.argtypes = [ctxp, dp]
+    L.ccx_energy_abc.restype = ctypes.c_int
     L.ccx_last_timing.argtypes = [ctxp, dp, dp]
     L.ccx_last_timing.restype = ctypes.c_int
     L.ccx_work_stats.argtypes = [ctxp, ctypes.POINTER(i64), dp]

@@ -35,6 +35,12 @@ int ccx_hip_finalize(const double *e_part, const double *v1_part, const int *tup
 int ccx_hip_t1wt3(const double *w, const double *voe6, double dijk, const double *ep, int nu, double *x3,
                   double *v1p, void *stream);
 
+/* virtual-triple view (ccx_abc.hip): trip holds (a, b, c, kind) per triple, 0-based */
+int ccx_hip_w_abc(const void *axr, const void *bx, const int *trip, int ntrip, int no, int nu, int nb, int kp,
+                  double *w, void *stream);
+int ccx_hip_x_abc(const double *w, const int *trip, int ntrip, int no, const double *eh, const double *ep,
+                  double *x, void *stream);
+
 /* adjoint (ccx_adjoint.hip) */
 int ccx_hip_adj_lds_bytes(void);
 int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const double *voep, const double *eh,

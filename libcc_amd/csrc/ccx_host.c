@@ -420,6 +420,101 @@ fail:
     return rc;
 }
 
+/* ------------------------------------------------------------------ virtual triples */
+
+/* The virtual triples a >= b >= c, not all equal, in the order a > b > c (a outermost), then
+ * (a,b,b) and (a,a,b) for each a > b; kind 1, 2, 3 as for occupied tuples.  Writes entries
+ * [first, first + n) of that list into out (4 ints each) and returns the list length. */
+static int64_t abc_triples(int64_t nu, int64_t first, int64_t n, int *out)
+{
+    int64_t pos = 0;
+#define ABC_PUT(A, B, C, KIND)                                                                 \
+    do {                                                                                      \
+        if (pos >= first && pos < first + n) {                                                \
+            int *q = out + 4 * (pos - first);                                                 \
+            q[0] = (int)(A); q[1] = (int)(B); q[2] = (int)(C); q[3] = (KIND);                 \
+        }                                                                                     \
+        ++pos;                                                                                \
+    } while (0)
+    for (int64_t a = 0; a < nu; ++a)
+        for (int64_t b = 0; b < a; ++b) {
+            if (!out || pos + b <= first) { pos += b; continue; }   /* count or skip a whole c row */
+            for (int64_t c = 0; c < b; ++c) ABC_PUT(a, b, c, 1);
+        }
+    for (int64_t a = 0; a < nu; ++a)
+        for (int64_t b = 0; b < a; ++b) {
+            ABC_PUT(a, b, b, 2);
+            ABC_PUT(a, a, b, 3);
+        }
+#undef ABC_PUT
+    return pos;
+}
+
+int ccx_w_abc(ccx_ctx *c, int64_t a, int64_t b, int64_t cc, double *v3)
+{
+    if (!c || !v3) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    if (a < 1 || b < 1 || cc < 1 || a > c->nu || b > c->nu || cc > c->nu) return CCX_ERR_ARG;
+    const size_t no3 = (size_t)(c->no * c->no * c->no);
+    int rc = CCX_OK;
+    int *d_trip = NULL;
+    double *d_w = NULL;
+    int trip[4] = {(int)(a - 1), (int)(b - 1), (int)(cc - 1), 1};
+    DALLOC(d_trip, 4);
+    DALLOC(d_w, no3);
+    if (ccx_hip_memcpy_h2d(d_trip, trip, sizeof(trip), c->stream) != 0 ||
+        ccx_hip_w_abc(c->d_axr, c->d_bx, d_trip, 1, (int)c->no, (int)c->nu, (int)c->nb, (int)c->kp, d_w,
+                      c->stream) != 0 ||
+        ccx_hip_memcpy_d2h(v3, d_w, sizeof(double) * no3, c->stream) != 0)
+        rc = CCX_ERR_DEVICE;
+fail:
+    ccx_hip_free(d_trip); ccx_hip_free(d_w);
+    return rc;
+}
+
+int ccx_energy_abc(ccx_ctx *c, double *etd)
+{
+    if (!c || !etd) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    const int64_t no3 = c->no * c->no * c->no;
+    const int64_t ntrip = abc_triples(c->nu, 0, 0, NULL);
+    int64_t batch = ((int64_t)256 << 20) / (no3 * (int64_t)sizeof(double));
+    if (batch < 1) batch = 1;
+    if (batch > 65536) batch = 65536;
+    if (batch > ntrip) batch = ntrip;
+    int rc = CCX_OK;
+    int *h_trip = NULL, *d_trip = NULL;
+    double *d_w = NULL, *d_x = NULL, *h_x = NULL;
+    double sum = 0.0;
+    if (ntrip > 0) {
+        h_trip = (int *)malloc(sizeof(int) * 4 * (size_t)batch);
+        h_x = (double *)malloc(sizeof(double) * (size_t)batch);
+        if (!h_trip || !h_x) { rc = CCX_ERR_ALLOC; goto fail; }
+        DALLOC(d_trip, 4 * batch);
+        DALLOC(d_w, no3 * batch);
+        DALLOC(d_x, batch);
+    }
+    /* batches in list order; the partials of each batch are added on the host in order */
+    for (int64_t first = 0; first < ntrip; first += batch) {
+        const int64_t n = ntrip - first < batch ? ntrip - first : batch;
+        abc_triples(c->nu, first, n, h_trip);
+        if (ccx_hip_memcpy_h2d(d_trip, h_trip, sizeof(int) * 4 * (size_t)n, c->stream) != 0 ||
+            ccx_hip_w_abc(c->d_axr, c->d_bx, d_trip, (int)n, (int)c->no, (int)c->nu, (int)c->nb, (int)c->kp, d_w,
+                          c->stream) != 0 ||
+            ccx_hip_x_abc(d_w, d_trip, (int)n, (int)c->no, c->d_eh, c->d_ep, d_x, c->stream) != 0 ||
+            ccx_hip_memcpy_d2h(h_x, d_x, sizeof(double) * (size_t)n, c->stream) != 0) {
+            rc = CCX_ERR_DEVICE;
+            goto fail;
+        }
+        for (int64_t t = 0; t < n; ++t) sum += h_x[t];
+    }
+    *etd = sum;
+fail:
+    free(h_trip); free(h_x);
+    ccx_hip_free(d_trip); ccx_hip_free(d_w); ccx_hip_free(d_x);
+    return rc;
+}
+
 int ccx_last_timing(const ccx_ctx *c, double *kernel_ms, double *total_ms)
 {
     if (!c) return CCX_ERR_ARG;

@@ -124,6 +124,19 @@ class TriplesContext:
         _check(lib().ccx_tuple_w(self._ctx, i, j, k, w.ctypes.data_as(_dp)), "ccx_tuple_w")
         return w
 
+    def w_abc(self, a: int, b: int, c: int) -> np.ndarray:
+        """Occupied cube v3(i,j,k) = W_ijk(a,b,c) of a 1-based virtual triple, v3(i,i,i) = 0
+        (the cube of the triples-abc prototype, src/triples-abc/abc_tuple.f90:1-81)."""
+        v3 = np.zeros((self.no, self.no, self.no), order="F")
+        _check(lib().ccx_w_abc(self._ctx, a, b, c, v3.ctypes.data_as(_dp)), "ccx_w_abc")
+        return v3
+
+    def energy_abc(self) -> float:
+        """E[T] re-summed over virtual triples (ccx_energy_abc); a cross-check route."""
+        etd = ctypes.c_double()
+        _check(lib().ccx_energy_abc(self._ctx, ctypes.byref(etd)), "ccx_energy_abc")
+        return etd.value
+
     def timing(self) -> tuple:
         k, t = ctypes.c_double(), ctypes.c_double()
         _check(lib().ccx_last_timing(self._ctx, ctypes.byref(k), ctypes.byref(t)), "ccx_last_timing")

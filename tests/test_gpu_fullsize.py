@@ -8,6 +8,13 @@ properties the mathematics fixes:
   order, and permutes v1 the same way.  Relabelling the virtuals moves every element to
   other 16-blocks and block orbits, so it exercises the whole block-orbit bookkeeping.
 
+The reduced forms (FP32, split BF16x3) are held to the same two properties at this shape:
+their power-of-two scaling is bitwise too (FP32 products of exactly doubled operands, and the
+split form's slab scales are powers of two, so its slices are exactly halved or kept), and
+they agree with the FP64 form to 1e-9 relative on E(T).  Their adjoints (W recomputed in the
+reduced form, the rest FP64) are within 1e-6 of the FP64 gradients in the relative Frobenius
+norm of every array.
+
 Inputs are the seeded generator's arrays built on the device (libcc_amd.synth)."""
 
 import numpy as np
@@ -83,3 +90,36 @@ def test_relabelling_invariance(full, which):
     # v1(a, i) of the relabelled problem = v1(p_vir[a], p_occ[i]) of the original
     want = ref["v1"][np.ix_(p_vir, p_occ)]
     np.testing.assert_allclose(r["v1"], want, rtol=0, atol=1e-10 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
+def test_reduced_form_at_full_size(full, form):
+    torch, inp, ctx, ref = full
+    print(f"full size form={form}", flush=True)
+    r = ctx.energy(form)
+    assert abs(r["e"] - ref["e"]) <= 1e-9 * abs(ref["e"]), (r["e"], ref["e"])
+    np.testing.assert_allclose(r["v1"], ref["v1"], rtol=0, atol=1e-5 * np.abs(ref["v1"]).max())
+    sc = dict(inp, vvvo=inp["vvvo"] * 2.0, vm=inp["vm"] * 2.0)
+    ctx.load(sc)
+    r2 = ctx.energy(form)
+    ctx.load(inp)
+    del sc
+    assert r2["etd"] == 4.0 * r["etd"]
+    assert r2["ets"] == 2.0 * r["ets"]
+    assert np.array_equal(r2["v1"], 2.0 * r["v1"])
+
+
+def test_reduced_adjoints_at_full_size(full):
+    torch, inp, ctx, ref = full
+    print("full size adjoint fp64", flush=True)
+    g64 = ctx.gradient(device=True)
+    assert abs(g64["e"] - ref["e"]) <= 1e-12 * abs(ref["e"])
+    for form in ("fp32", "bf16x3"):
+        print(f"full size adjoint {form}", flush=True)
+        g = ctx.gradient(device=True, form=form)
+        for k in ("t1", "t2", "vm", "voe", "vvvo"):
+            err = float(torch.linalg.norm(g[k] - g64[k]) / torch.linalg.norm(g64[k]))
+            assert err <= 1e-6, (form, k, err)
+        del g
+    del g64
+    torch.cuda.empty_cache()

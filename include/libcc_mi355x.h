@@ -85,7 +85,8 @@ int ccx_tuple_w(ccx_ctx *ctx, int64_t i, int64_t j, int64_t k, double *w);
  * the triples-abc prototype abc_tuple_formv3(a,b,c,...,v3) assembles from 12 GEMMs and
  * zerot3 (src/triples-abc/abc_tuple.f90:1-81).  That routine itself is not exported: it is
  * never called, its array arguments are declared with sizes its GEMMs do not use, and no
- * caller defines what its slices hold. */
+ * caller defines what its slices hold.  no^3 above 2^30 returns CCX_ERR_ARG (also for
+ * ccx_energy_abc). */
 int ccx_w_abc(ccx_ctx *ctx, int64_t a, int64_t b, int64_t c, double *v3);
 
 /* E[T] (etd) re-summed over virtual triples: sum over a>b>c of X_abc plus half the sum of

@@ -422,6 +422,9 @@ fail:
 
 /* ------------------------------------------------------------------ virtual triples */
 
+/* The virtual-triple kernels index an occupied cube with 32-bit ints. */
+#define ABC_MAX_CUBE ((int64_t)1 << 30)
+
 /* The virtual triples a >= b >= c, not all equal, in the order a > b > c (a outermost), then
  * (a,b,b) and (a,a,b) for each a > b; kind 1, 2, 3 as for occupied tuples.  Writes entries
  * [first, first + n) of that list into out (4 ints each) and returns the list length. */
@@ -455,6 +458,7 @@ int ccx_w_abc(ccx_ctx *c, int64_t a, int64_t b, int64_t cc, double *v3)
     if (!c || !v3) return CCX_ERR_ARG;
     if (!c->loaded) return CCX_ERR_STATE;
     if (a < 1 || b < 1 || cc < 1 || a > c->nu || b > c->nu || cc > c->nu) return CCX_ERR_ARG;
+    if (c->no * c->no * c->no > ABC_MAX_CUBE) return CCX_ERR_ARG;
     const size_t no3 = (size_t)(c->no * c->no * c->no);
     int rc = CCX_OK;
     int *d_trip = NULL;
@@ -477,6 +481,7 @@ int ccx_energy_abc(ccx_ctx *c, double *etd)
     if (!c || !etd) return CCX_ERR_ARG;
     if (!c->loaded) return CCX_ERR_STATE;
     const int64_t no3 = c->no * c->no * c->no;
+    if (no3 > ABC_MAX_CUBE) return CCX_ERR_ARG;
     const int64_t ntrip = abc_triples(c->nu, 0, 0, NULL);
     int64_t batch = ((int64_t)256 << 20) / (no3 * (int64_t)sizeof(double));
     if (batch < 1) batch = 1;

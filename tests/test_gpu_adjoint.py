@@ -119,6 +119,24 @@ def test_autograd_function_on_device_tensors():
         np.testing.assert_allclose(gk.cpu().numpy(), 3.0 * want[k], rtol=1e-13, atol=1e-300)
 
 
+@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
+def test_autograd_function_reduced_form(form):
+    """The same Function with W recomputed in a reduced form: both the 7-argument call and
+    the form argument work, and the gradients stay within 1e-6 of FP64 (Frobenius)."""
+    import torch
+    from libcc_amd.autograd import TriplesEnergy
+    inp = synth.make_inputs(4, 20, 9)
+    dev = {k: torch.tensor(np.ascontiguousarray(inp[k]), dtype=torch.float64, device="cuda",
+                           requires_grad=k in KEYS) for k in ("eh", "ep") + KEYS}
+    e64 = TriplesEnergy.apply(dev["eh"], dev["ep"], *(dev[k] for k in KEYS))
+    g64 = torch.autograd.grad(e64, [dev[k] for k in KEYS])
+    e = TriplesEnergy.apply(dev["eh"], dev["ep"], *(dev[k] for k in KEYS), form)
+    grads = torch.autograd.grad(e, [dev[k] for k in KEYS])
+    assert abs(float(e.detach()) - float(e64.detach())) <= 1e-6 * abs(float(e64.detach()))
+    for k, gk, gw in zip(KEYS, grads, g64):
+        assert float(torch.linalg.norm(gk - gw) / torch.linalg.norm(gw)) <= 1e-6, k
+
+
 def test_gradient_device_outputs_match_host():
     inp = synth.make_inputs(4, 20, 9)
     ctx = TriplesContext(4, 20)

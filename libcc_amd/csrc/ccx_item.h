@@ -157,6 +157,10 @@ struct ItemCtx {
     int ys[6], zs[6];   // BX slot -> ordered occupied pair
 };
 
+// Entry r (0..2) of a three-entry register array by selects.  A run-time index into the item
+// context would place it in scratch memory and serialise every read behind a scratch load.
+__device__ __forceinline__ int pick3(const int (&v)[3], int r) { return r == 0 ? v[0] : r == 1 ? v[1] : v[2]; }
+
 // Fragment base pointers (wave-uniform) of one phase for this wave's two batches; a
 // fragment of K step kb is base[kb*64 + lane].
 template <class T>

@@ -223,8 +223,8 @@ __device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], c
         const int qr = sb / 6, bpair = sb % 6;
         const int y = qr >> 1, z = (qr & 1) ? (y == 2 ? 1 : 2) : (y == 0 ? 1 : 0);
         const int ry = bpair >> 1, rz = (bpair & 1) ? (ry == 2 ? 1 : 2) : (ry == 0 ? 1 : 0);
-        const double *src = g.voe + (((long long)it.tpos[y] * g.no + it.tpos[z]) * g.nup + it.blk[ry] * TB + row) * g.nup
-                            + it.blk[rz] * TB + quad * 4;
+        const double *src = g.voe + (((long long)pick3(it.tpos, y) * g.no + pick3(it.tpos, z)) * g.nup
+                                     + pick3(it.blk, ry) * TB + row) * g.nup + pick3(it.blk, rz) * TB + quad * 4;
         double *d = lds + sb * VBLK + row * VSTR + quad * 4;
         d[0] = src[0]; d[1] = src[1]; d[2] = src[2]; d[3] = src[3];
     }

@@ -206,9 +206,11 @@ __global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__rest
 // ---------------------------------------------------------------------------------
 namespace {
 
-// FK: 0 FP64 slabs, 1 FP32 slabs (16 K per fragment load), 2 split BF16 slices
+// FK: 0 FP64 slabs, 1 FP32 slabs (16 K per fragment load), 2 split BF16 slices.  Always
+// inlined: as a called function every item paid a call frame and callee-saved register
+// spills in scratch memory (0.06 TB of writes per launch at no=24, nu=216).
 template <int MODE, class T, int FK>
-__device__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
+__device__ __forceinline__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
                         int tkind, int mult, int wave, int lane, int tid)
 {
     typename Mfma<T>::acc_t acc[6][BPW];

@@ -75,6 +75,15 @@ int ccx_load_device(ccx_ctx *ctx, const double *eh, const double *ep, const doub
  * launch.  v1 may be NULL; v1_on_device selects whether v1 is a device pointer. */
 int ccx_energy(ccx_ctx *ctx, int form, double *ets, double *etd, double *v1, int v1_on_device);
 
+/* ccx_energy over a subset of the tuples: n tuples (i,j,k), 1-based, i >= j >= k and not all
+ * equal, three int64 each in ijk.  A tuple with i > j > k carries weight 1, (i,j,j) and (i,i,j)
+ * weight 1/2, as in the full set (src/triples-ijk/t1wt3.f90:98-102, t3squa_acc.f:50-54), so the
+ * energies and v1 are the partial sums the reference's driver accumulates over those tuples
+ * (e.g. one tuple's ddcc_t_ijk_acc contribution, src/triples-ijk/ddcc_t_ijk_big.f:1-21).  At most
+ * as many tuples as the full set holds. */
+int ccx_energy_tuples(ccx_ctx *ctx, int form, int64_t n, const int64_t *ijk, double *ets, double *etd, double *v1,
+                      int v1_on_device);
+
 /* W_ijk(a,b,c) cube (nu^3, column-major, host) of one tuple, 1-based i,j,k, as the
  * reference's ijk_tuple leaves it in V3 (before the W(a,a,a)=0 step it is identical
  * off the diagonal). */

@@ -52,6 +52,8 @@ def lib() -> ctypes.CDLL:
     L.ccx_load_device.restype = ctypes.c_int
     L.ccx_energy.argtypes = [ctxp, ctypes.c_int, dp, dp, vp, ctypes.c_int]
     L.ccx_energy.restype = ctypes.c_int
+    L.ccx_energy_tuples.argtypes = [ctxp, ctypes.c_int, i64, ctypes.POINTER(i64), dp, dp, vp, ctypes.c_int]
+    L.ccx_energy_tuples.restype = ctypes.c_int
     L.ccx_tuple_w.argtypes = [ctxp, i64, i64, i64, dp]
     L.ccx_tuple_w.restype = ctypes.c_int
     L.ccx_w_abc.argtypes = [ctxp, i64, i64, i64, dp]

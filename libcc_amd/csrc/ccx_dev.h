@@ -18,6 +18,8 @@ const char *ccx_hip_last_error(void);
 int ccx_hip_lds_bytes(void);
 
 int ccx_hip_unpack_vvvo(const double *packed, double *full, int no, int nu, void *stream);
+int ccx_hip_nonfinite(const double *const *p, const size_t *n, int narr, unsigned int *d_flag, int *nonfinite,
+                      void *stream);
 int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *vm, const double *voe, void *axr,
                  void *axc, void *bx, double *voep, int no, int nu, int nup, int kp, void *stream);
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,

@@ -30,6 +30,7 @@ struct ccx_ctx {
     int split_ready;
     int *d_tuples, *d_blocks;
     unsigned int *d_counter;
+    unsigned int *d_flag;               /* NaN/Inf flag of ccx_load_device */
     double *d_epart, *d_v1part, *d_v1t, *d_v1, *d_out;
     int *h_tuples, *h_blocks;
     double kernel_ms, total_ms;
@@ -158,6 +159,7 @@ int ccx_destroy(ccx_ctx *c)
 {
     if (!c) return CCX_ERR_ARG;
     free_dev(c);
+    ccx_hip_free(c->d_flag);
     for (int q = 0; q < 4; ++q) ccx_hip_event_destroy(c->ev[q]);
     free(c->h_tuples);
     free(c->h_blocks);
@@ -243,6 +245,17 @@ int ccx_load_device(ccx_ctx *c, const double *eh, const double *ep, const double
 {
     if (!c || !eh || !ep || !t1 || !t2 || !vm || !voe || !vvvo) return CCX_ERR_ARG;
     if (vvvo_layout != 0 && vvvo_layout != 1) return CCX_ERR_ARG;
+    {   /* the same contract as ccx_load_host: NaN/Inf anywhere is CCX_ERR_INPUT */
+        const int64_t no = c->no, nu = c->nu;
+        const double *arr[7] = {eh, ep, t1, t2, vm, voe, vvvo};
+        const size_t n[7] = {(size_t)no, (size_t)nu, (size_t)(no * nu), (size_t)(nu * nu * no * no),
+                             (size_t)(no * nu * no * no), (size_t)(nu * nu * no * no),
+                             vvvo_layout == 0 ? (size_t)(nu * nu * nu * no) : (size_t)(nu * (nu + 1) / 2 * nu * no)};
+        int bad = 0;
+        if (!c->d_flag && ccx_hip_malloc((void **)&c->d_flag, sizeof(unsigned int)) != 0) return CCX_ERR_ALLOC;
+        if (ccx_hip_nonfinite(arr, n, 7, c->d_flag, &bad, c->stream) != 0) return CCX_ERR_DEVICE;
+        if (bad) return CCX_ERR_INPUT;
+    }
     if (vvvo_layout == 0) return build_from_device(c, eh, ep, t1, t2, vm, voe, vvvo);
     double *full = NULL;
     if (ccx_hip_malloc((void **)&full, sizeof(double) * (size_t)(c->nu * c->nu * c->nu * c->no)) != 0)
@@ -346,12 +359,12 @@ static int ensure_split(ccx_ctx *c)
     return CCX_OK;
 }
 
-int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v1_on_device)
+/* One persistent launch over the tuple table d_tup (ntup entries of 4 ints) and the
+ * deterministic reductions; energies to ets and etd, v1 (nu,no) to v1 when non-null. */
+static int energy_run(ccx_ctx *c, int form, const int *d_tup, int64_t ntup, double *ets, double *etd, double *v1,
+                      int v1_on_device)
 {
-    if (!c) return CCX_ERR_ARG;
-    if (!c->loaded) return CCX_ERR_STATE;
-    if (form != CCX_FORM_FP64 && form != CCX_FORM_FP32 && form != CCX_FORM_BF16X3) return CCX_ERR_FORM;
-    const int64_t nitems = c->ntup * c->nblk;
+    const int64_t nitems = ntup * c->nblk;
     double host_out[2] = {0.0, 0.0};
     if (form == CCX_FORM_FP32) {
         int rc = ensure_f32(c);
@@ -368,14 +381,14 @@ int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v
     if (ccx_hip_event_record(c->ev[0], c->stream) != 0) return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[1], c->stream) != 0) return CCX_ERR_DEVICE;
     if (nitems > 0 &&
-        ccx_hip_triples(form, axr, axc, bx, c->d_voep, c->d_eh, c->d_ep, c->d_tuples, c->d_blocks, c->d_epart,
+        ccx_hip_triples(form, axr, axc, bx, c->d_voep, c->d_eh, c->d_ep, d_tup, c->d_blocks, c->d_epart,
                         c->d_v1part, NULL, c->d_counter, (int)c->no, (int)c->nu, (int)c->nup, (int)c->kp,
-                        (int)c->ntup, (int)c->nblk, grid_size(c, nitems), split,
+                        (int)ntup, (int)c->nblk, grid_size(c, nitems), split,
                         form == CCX_FORM_FP32 ? (int)c->kpf : (int)c->kps, c->split_scale, c->stream) != 0)
         return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[2], c->stream) != 0) return CCX_ERR_DEVICE;
-    if (ccx_hip_finalize(c->d_epart, c->d_v1part, c->d_tuples, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,
-                         (int)c->no, (int)c->nu, (int)c->ntup, (int)c->nblk, c->stream) != 0)
+    if (ccx_hip_finalize(c->d_epart, c->d_v1part, d_tup, c->d_blocks, c->d_t1, c->d_v1t, c->d_v1, c->d_out,
+                         (int)c->no, (int)c->nu, (int)ntup, (int)c->nblk, c->stream) != 0)
         return CCX_ERR_DEVICE;
     if (ccx_hip_event_record(c->ev[3], c->stream) != 0) return CCX_ERR_DEVICE;
     if (ccx_hip_memcpy_d2h(host_out, c->d_out, sizeof(double) * 2, c->stream) != 0) return CCX_ERR_DEVICE;
@@ -393,6 +406,45 @@ int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v
     if (etd) *etd = host_out[0];
     if (ets) *ets = host_out[1];
     return CCX_OK;
+}
+
+static int form_known(int form)
+{
+    return form == CCX_FORM_FP64 || form == CCX_FORM_FP32 || form == CCX_FORM_BF16X3;
+}
+
+int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v1_on_device)
+{
+    if (!c) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    if (!form_known(form)) return CCX_ERR_FORM;
+    return energy_run(c, form, c->d_tuples, c->ntup, ets, etd, v1, v1_on_device);
+}
+
+int ccx_energy_tuples(ccx_ctx *c, int form, int64_t n, const int64_t *ijk, double *ets, double *etd, double *v1,
+                      int v1_on_device)
+{
+    if (!c || (n > 0 && !ijk) || n < 0) return CCX_ERR_ARG;
+    if (!c->loaded) return CCX_ERR_STATE;
+    if (!form_known(form)) return CCX_ERR_FORM;
+    if (n > c->ntup) return CCX_ERR_ARG;   /* the partial buffers hold one full tuple set */
+    int *h = (int *)malloc(sizeof(int) * 4 * (size_t)(n > 0 ? n : 1));
+    if (!h) return CCX_ERR_ALLOC;
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t i = ijk[3 * t], j = ijk[3 * t + 1], k = ijk[3 * t + 2];
+        if (i < 1 || k < 1 || i > c->no || !(i >= j && j >= k) || (i == j && j == k)) { free(h); return CCX_ERR_ARG; }
+        h[4 * t] = (int)(i - 1); h[4 * t + 1] = (int)(j - 1); h[4 * t + 2] = (int)(k - 1);
+        h[4 * t + 3] = (i > j && j > k) ? 1 : (j == k ? 2 : 3);
+    }
+    int *d = NULL;
+    int rc = CCX_OK;
+    if (ccx_hip_malloc((void **)&d, sizeof(int) * 4 * (size_t)(n > 0 ? n : 1)) != 0) { free(h); return CCX_ERR_ALLOC; }
+    if (n > 0 && ccx_hip_memcpy_h2d(d, h, sizeof(int) * 4 * (size_t)n, c->stream) != 0) rc = CCX_ERR_DEVICE;
+    if (rc == CCX_OK) rc = energy_run(c, form, d, n, ets, etd, v1, v1_on_device);
+    ccx_hip_sync(c->stream);
+    ccx_hip_free(d);
+    free(h);
+    return rc;
 }
 
 int ccx_tuple_w(ccx_ctx *c, int64_t i, int64_t j, int64_t k, double *w)
@@ -828,15 +880,13 @@ static int legacy_ctx(void)
     return CCX_OK;
 }
 
-/* TRPOSE(t1,tmp,no,nu,1): t1(no,nu) -> t1(nu,no) in place (src/triples-ijk/common_functions.f:3-41) */
-static void transpose_inplace(double *a, int64_t n, int64_t m)
+/* TRPOSE(t1,tmp,no,nu,1): t1(no,nu) -> t1(nu,no) in place (src/triples-ijk/common_functions.f:3-41),
+ * through the caller's scratch b of n*m doubles */
+static void transpose_inplace(double *a, double *b, int64_t n, int64_t m)
 {
-    double *b = (double *)malloc(sizeof(double) * (size_t)(n * m));
-    if (!b) return;
     for (int64_t j = 0; j < m; ++j)
         for (int64_t i = 0; i < n; ++i) b[j + m * i] = a[i + n * j];
     memcpy(a, b, sizeof(double) * (size_t)(n * m));
-    free(b);
 }
 
 static void legacy_triples(int64_t handle, double *eh, double *ep, double *v1, double *t1, double *t2,
@@ -848,17 +898,22 @@ static void legacy_triples(int64_t handle, double *eh, double *ep, double *v1, d
         if (g_reg[q].handle == handle) { vvvo = g_reg[q].ptr; layout = g_reg[q].layout; }
     if (!vvvo) { g_status = CCX_ERR_STATE; return; }
     if (!eh || !ep || !v1 || !t1 || !t2 || !vm || !voe) { g_status = CCX_ERR_ARG; return; }
+    /* scratch for the closing t1 transpose, taken first: if it is not there nothing is done */
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)(g_no > 0 && g_nu > 0 ? g_no * g_nu : 1));
+    if (!tmp) { g_status = CCX_ERR_ALLOC; return; }
     int rc = legacy_ctx();
     if (rc == CCX_OK) rc = ccx_load_host(g_ctx, eh, ep, t1, t2, vm, voe, vvvo, (int)layout);
     double ets = 0.0, etd = 0.0;
     if (rc == CCX_OK) rc = ccx_energy(g_ctx, CCX_FORM_FP64, &ets, &etd, v1, 0);
     g_status = rc;
-    if (rc != CCX_OK) return;
-    g_ets = ets;
-    g_etd = etd;
-    if (gms_ets) *gms_ets = ets;
-    if (gms_etd) *gms_etd = etd;
-    transpose_inplace(t1, g_no, g_nu);
+    if (rc == CCX_OK) {
+        g_ets = ets;
+        g_etd = etd;
+        if (gms_ets) *gms_ets = ets;
+        if (gms_etd) *gms_etd = etd;
+        transpose_inplace(t1, tmp, g_no, g_nu);
+    }
+    free(tmp);
 }
 
 void cc_triples_acc_(double *eh, double *ep, double *v1, double *t1, double *t2, double *v3, double *t3,

@@ -523,6 +523,15 @@ __global__ void __launch_bounds__(256) t1wt3_kernel(const double *__restrict__ w
 
 __global__ void zero_counter_kernel(unsigned int *c) { *c = 0u; }
 
+// flag |= 1 when any of the n doubles is NaN or infinite (one atomic per wave at most)
+__global__ void nonfinite_kernel(const double *__restrict__ p, long long n, unsigned int *__restrict__ flag)
+{
+    int bad = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        bad |= !isfinite(p[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 // FP32 slabs from the FP64 ones in the FP32 sweep's fragment order: AX [panel][outer][K block
 // of 16][lane][4], BX [panel][K block][lane][4]; lane l holds inner index l & 15, element j
 // is at K = 16 kb + 4 j + (l >> 4).  BX = 1: the source is the BX slab [panel][K][16 gamma].
@@ -740,6 +749,22 @@ int ccx_hip_pack_f32v(const double *axr, const double *axc, const double *bx, fl
     pack_f32v_kernel<0><<<grid_for(nax), 256, 0, s>>>(axc, axc32, (long long)no * nb * nb, kp, kpf);
     pack_f32v_kernel<1><<<grid_for(nbx), 256, 0, s>>>(bx, bx32, (long long)no * no * nb, kp, kpf);
     return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// *nonfinite = 1 when any element of the arrays p[0..narr) (n[q] doubles each) is NaN or infinite
+int ccx_hip_nonfinite(const double *const *p, const size_t *n, int narr, unsigned int *d_flag, int *nonfinite,
+                      void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    zero_counter_kernel<<<1, 1, 0, s>>>(d_flag);
+    for (int q = 0; q < narr; ++q)
+        if (n[q] > 0) nonfinite_kernel<<<grid_for((long long)n[q]), 256, 0, s>>>(p[q], (long long)n[q], d_flag);
+    unsigned int h = 0;
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&h, d_flag, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return 1;
+    *nonfinite = h != 0;
+    return 0;
 }
 
 int ccx_hip_unpack_vvvo(const double *packed, double *full, int no, int nu, void *stream)

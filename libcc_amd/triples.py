@@ -26,6 +26,35 @@ def _host(a) -> np.ndarray:
     return np.asfortranarray(a)
 
 
+def _fortran_buffer_np(a, shape: tuple, name: str) -> np.ndarray:
+    """The Fortran-ordered float64 buffer of a host array given in its logical shape or as a
+    1-D buffer of the right length."""
+    a = np.asarray(a)
+    n = int(np.prod(shape))
+    if a.ndim == 1 and len(shape) > 1:
+        if a.size != n:
+            raise ValueError(f"{name}: 1-D buffer of {a.size} elements, expected {n} for {shape}")
+        return np.ascontiguousarray(a, dtype=np.float64)
+    if tuple(a.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(a.shape)}, expected {tuple(shape)} (or a 1-D buffer of {n})")
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _fortran_buffer_t(t, shape: tuple, name: str):
+    """The Fortran-ordered buffer (1-D, contiguous) of a device tensor in its logical shape, or
+    the tensor itself when it already is a contiguous 1-D buffer of the right length."""
+    n = int(np.prod(shape))
+    if t.dim() == 1 and len(shape) > 1:
+        if t.numel() != n:
+            raise ValueError(f"{name}: 1-D buffer of {t.numel()} elements, expected {n} for {shape}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name}: a 1-D buffer must be contiguous")
+        return t
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)} (or a 1-D buffer of {n})")
+    return t.permute(*reversed(range(t.dim()))).contiguous().reshape(-1)
+
+
 def _check(status: int, what: str) -> None:
     if status != 0:
         raise CcxError(status, what)
@@ -51,28 +80,55 @@ class TriplesContext:
         except Exception:
             pass
 
+    def shapes(self, packed_vvvo: bool = False) -> dict:
+        """Logical shapes of the inputs in the reference layouts."""
+        no, nu = self.no, self.nu
+        return {"eh": (no,), "ep": (nu,), "t1": (no, nu), "t2": (nu, nu, no, no), "vm": (no, nu, no, no),
+                "voe": (nu, nu, no, no),
+                "vvvo": (nu * (nu + 1) // 2, nu * no) if packed_vvvo else (nu, nu, nu, no)}
+
     def load(self, inp: dict, packed_vvvo: bool = False) -> None:
-        """Upload inputs in the reference layouts (numpy, host) or use torch GPU tensors in place."""
+        """Upload inputs in the reference layouts (numpy, host) or use torch GPU tensors in place.
+
+        Every array is either in its logical shape (indexed as the reference indexes it, e.g.
+        t1[i, a] = T1(i,a), any strides) or a 1-D buffer holding the Fortran-ordered array; a
+        1-D tensor on the device must be contiguous.  Sizes, dtype (float64) and, for tensors,
+        the device (the current one) are checked before anything is handed to the library."""
         keys = ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")
-        first = inp["vvvo"]
+        shapes = self.shapes(packed_vvvo)
         layout = 1 if packed_vvvo else 0
-        if hasattr(first, "is_cuda") and first.is_cuda:
-            ptrs = []
+        if any(hasattr(inp[k], "is_cuda") and inp[k].is_cuda for k in keys):
+            import torch
+
+            dev = torch.cuda.current_device()
+            bufs = []
             for k in keys:
                 t = inp[k]
-                if t.dtype != __import__("torch").float64:
-                    raise TypeError(f"{k} must be float64")
-                ptrs.append(ctypes.c_void_p(t.data_ptr()))
-            self._keep = [inp[k] for k in keys]
-            _check(lib().ccx_load_device(self._ctx, *ptrs, layout), "ccx_load_device")
+                if not (hasattr(t, "is_cuda") and t.is_cuda):
+                    raise TypeError(f"{k}: every input must be a tensor on the GPU when one is")
+                if t.dtype != torch.float64:
+                    raise TypeError(f"{k} must be float64, got {t.dtype}")
+                if t.device.index != dev:
+                    raise ValueError(f"{k} is on cuda:{t.device.index}, the library uses cuda:{dev}")
+                bufs.append(_fortran_buffer_t(t, shapes[k], k))
+            self._keep = bufs
+            _check(lib().ccx_load_device(self._ctx, *[ctypes.c_void_p(b.data_ptr()) for b in bufs], layout),
+                   "ccx_load_device")
         else:
-            arrs = [_host(inp[k]) for k in keys]
+            arrs = [_fortran_buffer_np(inp[k], shapes[k], k) for k in keys]
             _check(lib().ccx_load_host(self._ctx, *[a.ctypes.data_as(_dp) for a in arrs], layout),
                    "ccx_load_host")
 
     def energy(self, form: str = "fp64", v1_device=None) -> dict:
         ets, etd = ctypes.c_double(), ctypes.c_double()
+        if form not in FORMS:
+            raise ValueError(f"unknown form {form!r}; one of {sorted(FORMS)}")
         if v1_device is not None:
+            import torch
+
+            if (not v1_device.is_cuda or v1_device.dtype != torch.float64 or not v1_device.is_contiguous()
+                    or v1_device.numel() != self.no * self.nu or v1_device.device.index != torch.cuda.current_device()):
+                raise ValueError("v1_device must be a contiguous float64 tensor of no*nu elements on the current GPU")
             _check(lib().ccx_energy(self._ctx, FORMS[form], ctypes.byref(ets), ctypes.byref(etd),
                                     ctypes.c_void_p(v1_device.data_ptr()), 1), "ccx_energy")
             v1 = v1_device
@@ -80,6 +136,17 @@ class TriplesContext:
             v1 = np.zeros((self.nu, self.no), order="F")
             _check(lib().ccx_energy(self._ctx, FORMS[form], ctypes.byref(ets), ctypes.byref(etd),
                                     ctypes.c_void_p(v1.ctypes.data), 0), "ccx_energy")
+        return {"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value, "v1": v1}
+
+    def energy_tuples(self, tuples, form: str = "fp64") -> dict:
+        """ccx_energy over the given 1-based (i, j, k) tuples (i >= j >= k, not all equal): the
+        partial E[T], E_ST and v1 the reference's driver accumulates over exactly those tuples."""
+        arr = np.ascontiguousarray(np.asarray(tuples, dtype=np.int64).reshape(-1, 3))
+        ets, etd = ctypes.c_double(), ctypes.c_double()
+        v1 = np.zeros((self.nu, self.no), order="F")
+        _check(lib().ccx_energy_tuples(self._ctx, FORMS[form], arr.shape[0],
+                                       arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(ets),
+                                       ctypes.byref(etd), ctypes.c_void_p(v1.ctypes.data), 0), "ccx_energy_tuples")
         return {"ets": ets.value, "etd": etd.value, "e": ets.value + etd.value, "v1": v1}
 
     def gradient(self, device: bool = False, form: str = "fp64") -> dict:

@@ -169,3 +169,62 @@ def grad_dense(inp: dict) -> dict:
     out = {k: np.asfortranarray(leaves[k].grad.numpy()) for k in leaves}
     out["e"] = float(e.detach())
     return out
+
+
+def grad_tuplewise(inp: dict, progress=None) -> dict:
+    """grad_dense evaluated one ordered occupied triple at a time, so that shapes whose dense
+    W (no^3 nu^3) does not fit in memory can be differentiated (no=8, nu=120: 7 GB dense).
+
+    E(T) = sum over ordered (i,j,k), not all equal, of X_ijk / 6 + 2 sum_abc T1(i,a) Z_ijk(a,b,c)
+    VOE(b,c,j,k), with W_ijk(a,b,c) = sum_s P_{x'}(y'), x'[s[m]] = (i,j,k)[m],
+    y'[s[m]] = (a,b,c)[m] and P_x = [sum_d VVVO(d,b,a,x0) T2(c,d,x1,x2) - sum_l T2(a,b,l,x0)
+    VM(l,c,x2,x1)], the per-triple slice of w_dense / _sym6.  Torch autograd (float64, CPU)
+    accumulates the gradient of every triple's energy into the same leaves."""
+    import torch
+
+    f64 = torch.float64
+    no, nu = inp["no"], inp["nu"]
+    leaves = {k: torch.tensor(np.ascontiguousarray(inp[k]), dtype=f64, requires_grad=True)
+              for k in ("t1", "t2", "vm", "voe", "vvvo")}
+    eh = torch.tensor(inp["eh"], dtype=f64)
+    ep = torch.tensor(inp["ep"], dtype=f64)
+    vsum = ep[:, None, None] + ep[None, :, None] + ep[None, None, :]
+    a = torch.arange(nu)
+    mask = torch.ones(nu, nu, nu, dtype=f64)
+    mask[a, a, a] = 0.0
+    etot = 0.0
+    vv, t2, vm, t1, voe = (leaves[k] for k in ("vvvo", "t2", "vm", "t1", "voe"))
+    n_done = 0
+    for i in range(no):
+        for j in range(no):
+            for k in range(no):
+                if i == j == k:
+                    continue
+                x = (i, j, k)
+                w = None
+                for s in _PERMS:
+                    xp = [0, 0, 0]
+                    for m in range(3):
+                        xp[s[m]] = x[m]
+                    p = (torch.einsum("dba,cd->abc", vv[:, :, :, xp[0]], t2[:, :, xp[1], xp[2]])
+                         - torch.einsum("abl,lc->abc", t2[:, :, :, xp[0]], vm[:, :, xp[2], xp[1]]))
+                    term = p.permute(*s)
+                    w = term if w is None else w + term
+                d = (eh[i] + eh[j] + eh[k]) - vsum
+
+                def vp(spec):
+                    return torch.einsum(spec + "->abc", w)
+
+                f = 8 * w - 4 * (vp("acb") + vp("cba") + vp("bac")) + 2 * (vp("bca") + vp("cab"))
+                e_d = (w * f / d * mask).sum() / 6.0
+                z = (2 * w - 2 * vp("bac") - vp("acb") + vp("bca")) / d
+                e_s = 2 * torch.einsum("a,abc,bc->", t1[i], z, voe[:, :, j, k])
+                e = e_d + e_s
+                e.backward()
+                etot += float(e.detach())
+                n_done += 1
+                if progress is not None:
+                    progress(n_done)
+    out = {k: np.asfortranarray(leaves[k].grad.numpy()) for k in leaves}
+    out["e"] = etot
+    return out

@@ -32,8 +32,10 @@ from oracle import ref  # noqa: E402
 
 CASES = [(3, 5, 7), (4, 6, 11), (5, 7, 3), (2, 9, 5), (6, 4, 21),
          (4, 36, 1001), (4, 87, 1002), (8, 120, 1003)]
+# (kind, i, j, k), 0-based: kind 0 the distinct tuple (i,j,k); kind 1 the tuple (i,j,j) and
+# kind 2 the tuple (i,i,j), both from (i, j) with i > j (k unused)
 TUPLES = [(0, 2, 1, 0), (0, 3, 2, 1), (0, 3, 1, 0), (1, 2, 1, 1), (1, 3, 0, 0),
-          (2, 2, 2, 1), (2, 3, 3, 0)]
+          (2, 2, 1, 0), (2, 3, 0, 0)]
 # TRANMD_SMP codes on the 3^4 array and on shapes where only the permuted extents agree
 CODES = (12, 13, 14, 23, 24, 34, 231, 312, 341, 413, 1234, 99)
 TRANMD = [((3, 3, 3, 3), c) for c in CODES] + [

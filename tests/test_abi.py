@@ -170,3 +170,17 @@ def test_no_gpu_is_reported_not_crashed():
         pytest.skip("a GPU is present")
     ctx = ctypes.c_void_p()
     assert lib().ccx_create(3, 5, ctypes.byref(ctx)) == 2
+
+
+def test_host_input_shapes_checked():
+    """The Python mirror turns logical-shape arrays and 1-D Fortran buffers into the same
+    buffer and rejects a wrong size before the library can read past it."""
+    from libcc_amd.triples import _fortran_buffer_np
+    a = np.arange(24.0).reshape(2, 3, 4, order="F")
+    assert np.array_equal(_fortran_buffer_np(a, (2, 3, 4), "x").ravel(order="K"), np.arange(24.0))
+    c = np.ascontiguousarray(a)
+    assert np.array_equal(_fortran_buffer_np(c, (2, 3, 4), "x").ravel(order="F"), a.ravel(order="F"))
+    assert np.array_equal(_fortran_buffer_np(a.ravel(order="F"), (2, 3, 4), "x"), np.arange(24.0))
+    for wrong in (np.zeros((2, 3, 5)), np.zeros(23), np.zeros((4, 3, 2))):
+        with pytest.raises(ValueError):
+            _fortran_buffer_np(wrong, (2, 3, 4), "x")

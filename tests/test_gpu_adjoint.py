@@ -7,6 +7,8 @@
 * FD:     central finite differences of the GPU FP64 energy, step 1e-5 relative to the
   RMS of each input, along 8 seeded random directions at no=8, nu=120 (the bar is a
   relative error <= 1e-6 of the directional derivative);
+* fixture: the FP64 dense-einsum gradient at no=8, nu=120 (tests/golden/grad_c3.npz),
+  directional derivatives and sampled entries;
 * the torch.autograd.Function wrapper, run-to-run bitwise determinism, and that
   splitting the tuple loop into batches changes nothing beyond rounding."""
 
@@ -103,6 +105,36 @@ def test_finite_differences_at_adjoint_size():
         print(f"direction: <g,d> = {lin:.12e}  FD = {fd:.12e}  rel = {rel:.2e}", flush=True)
     ctx.close()
     assert worst <= 1e-6, worst
+
+
+def test_gradient_matches_dense_fixture_at_adjoint_size():
+    """no=8, nu=120 against the FP64 dense-einsum gradient (tests/golden/grad_c3.npz, written by
+    tests/golden/make_grad_c3.py through oracle.dense.grad_tuplewise): the directional
+    derivative along 8 seeded directions per array, to 1e-10 of |grad| |direction|, and 64
+    sampled entries per array, to 1e-11 of the array's max |grad|."""
+    from golden.make_grad_c3 import DIR_SEED, NDIR, KEYS as GKEYS, direction  # noqa: F401
+
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "grad_c3.npz"), allow_pickle=False) as f:
+        fix = {k: f[k] for k in f.files}
+    no, nu, seed = synth.CONFIGS["c3"]
+    inp = synth.make_inputs(no, nu, seed)
+    g = _grad(inp)
+    assert abs(g["e"] - fix["e"][0]) <= 1e-12 * abs(fix["e"][0])
+    worst_dd = worst_s = 0.0
+    for q, k in enumerate(GKEYS):
+        gn = np.linalg.norm(g[k])
+        assert abs(gn - fix["gnorm"][q]) <= 1e-11 * fix["gnorm"][q], k
+        for n in range(NDIR):
+            d = direction(n, q, inp[k].shape)
+            got = float(np.sum(g[k] * d))
+            rel = abs(got - fix["dd"][n, q]) / (gn * np.linalg.norm(d))
+            worst_dd = max(worst_dd, rel)
+            assert rel <= 1e-10, (k, n, got, fix["dd"][n, q])
+        flat = g[k].ravel(order="F")[fix["idx_" + k]]
+        rel = np.abs(flat - fix["g_" + k]).max() / np.abs(fix["g_" + k]).max()
+        worst_s = max(worst_s, rel)
+        assert rel <= 1e-11, (k, rel)
+    print(f"dense-einsum gradient at no=8 nu=120: directional {worst_dd:.2e}, sampled entries {worst_s:.2e}")
 
 
 def test_autograd_function_on_device_tensors():

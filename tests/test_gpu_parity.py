@@ -2,7 +2,8 @@
 
 Golden vectors come from the reference's compiled per-tuple routines
 (tests/golden/golden.npz); the C restatement (oracle/port.c) covers cases the golden
-file does not hold.  Bar: |dE| <= 1e-9 Eh and 1e-12 relative, v1 to 1e-10 relative."""
+file does not hold.  Bar: |dE| <= 1e-9 Eh absolute for |E| below 2e3 Eh (every golden case
+up to no=4, nu=87), else 1e-12 relative; v1 to 1e-10 relative."""
 
 import ctypes
 
@@ -10,17 +11,31 @@ import numpy as np
 import pytest
 from conftest import GOLDEN_CASES
 
-from libcc_amd import lib, synth
+from libcc_amd import CcxError, lib, synth
 from libcc_amd.triples import TriplesContext, cc_triples
 from oracle import dense, port
 
 pytestmark = pytest.mark.gpu
 
 
-def _energy_ok(got_ets, got_etd, ets, etd):
-    scale = max(1.0, abs(etd))
-    assert abs(got_etd - etd) <= max(1e-9, 1e-12 * scale), (got_etd, etd)
-    assert abs(got_ets - ets) <= max(1e-9, 1e-12 * scale), (got_ets, ets)
+# Bar: |dE| <= 1e-9 Eh absolute on etd, ets and E(T) = ets + etd for every energy below
+# 2e3 Eh (no=4, nu=36: |E| = 9.0 Eh; no=4, nu=87: 1.1e3 Eh); above that the generator's
+# integrals make E so large that FP64 summation order alone moves it by more, and the bar is
+# 1e-12 relative (no=8, nu=120: |E| = 5.5e4 Eh).
+ABS_BAR = 1e-9
+
+
+def _energy_ok(got_ets, got_etd, ets, etd, absolute=None):
+    scale = abs(etd)
+    if absolute is None:
+        absolute = scale < 2e3
+    bar = ABS_BAR if absolute else 1e-12 * scale
+    d_etd, d_ets = abs(got_etd - etd), abs(got_ets - ets)
+    d_e = abs((got_etd + got_ets) - (etd + ets))
+    print(f"|dE| etd {d_etd:.3e} ets {d_ets:.3e} E(T) {d_e:.3e} Eh (bar {bar:.1e}, |E| {abs(etd + ets):.4e})")
+    assert d_etd <= bar, (got_etd, etd)
+    assert d_ets <= bar, (got_ets, ets)
+    assert d_e <= bar, (got_etd + got_ets, etd + ets)
 
 
 def _v1_ok(got, want):
@@ -183,6 +198,44 @@ def test_device_resident_inputs(golden):
     r = ctx.energy(v1_device=v1d)
     _energy_ok(r["ets"], r["etd"], *golden["e_4_87_1002"])
     _v1_ok(v1d.cpu().numpy().reshape(87, 4, order="F"), golden["v1_4_87_1002"])
+    ctx.close()
+
+
+def test_logical_shape_device_tensors_match_host(golden):
+    """torch.from_numpy(inp[k]).cuda() keeps each array's logical shape (Fortran strides), and a
+    C-contiguous tensor of the same logical shape holds the same values in another order: both
+    must give the numpy path's energy; wrong sizes are rejected before the library reads them."""
+    torch = pytest.importorskip("torch")
+    inp = synth.make_inputs(4, 36, 1001)
+    keys = ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")
+    ctx = TriplesContext(4, 36)
+    host = {k: inp[k] for k in keys}
+    ctx.load(host)
+    want = ctx.energy()
+    for make in (lambda a: torch.from_numpy(a).cuda(), lambda a: torch.tensor(np.ascontiguousarray(a)).cuda()):
+        ctx.load({k: make(inp[k]) for k in keys})
+        r = ctx.energy()
+        assert r["etd"] == want["etd"] and r["ets"] == want["ets"]
+        assert np.array_equal(r["v1"], want["v1"])
+    _energy_ok(want["ets"], want["etd"], *golden["e_4_36_1001"])
+    bad = {k: torch.from_numpy(inp[k]).cuda() for k in keys}
+    bad["t2"] = bad["t2"][:, :, :, :3]
+    with pytest.raises(ValueError):
+        ctx.load(bad)
+    with pytest.raises(ValueError):
+        ctx.load(dict(host, vvvo=synth.pack_vvvo(inp["vvvo"])))      # packed array, full layout asked
+    ctx.close()
+
+
+def test_nonfinite_device_input_rejected():
+    torch = pytest.importorskip("torch")
+    inp = synth.make_inputs(3, 5, 7)
+    dev = {k: torch.from_numpy(inp[k]).cuda() for k in ("eh", "ep", "t1", "t2", "vm", "voe", "vvvo")}
+    dev["vvvo"][1, 2, 3, 0] = float("inf")
+    ctx = TriplesContext(3, 5)
+    with pytest.raises(CcxError) as e:
+        ctx.load(dev)
+    assert e.value.status == 5          # CCX_ERR_INPUT, as from host memory
     ctx.close()
 
 

@@ -170,3 +170,72 @@ def test_task_decode_order():
     no = 6
     want = [(i, j, k) for i in range(no) for j in range(i) for k in range(j)]
     assert [port.task(t, no) for t in range(len(want))] == want
+
+
+@pytest.mark.parametrize("case", [(3, 5, 7), (4, 6, 11), (2, 9, 5)])
+def test_grad_voe_matches_dense_autograd(case):
+    """dense.grad_voe (the closed form 2 sum T1 Z) against torch autograd through the dense
+    energy; a dropped factor 2, a transposed (q,r) pair or Z read at (a,c,b) fails it."""
+    inp = synth.make_inputs(*case)
+    want = dense.grad_dense(inp)["voe"]
+    got = dense.grad_voe(inp)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-14 * np.abs(want).max())
+    # a plausible mistake: the (q,r) pair swapped
+    assert np.abs(np.swapaxes(got, 2, 3) - want).max() > 1e-3 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("case", [(3, 5, 7), (4, 6, 11), (2, 9, 5), (5, 3, 2)])
+def test_grad_tuplewise_matches_dense(case):
+    """The per-triple gradient (used for the no=8, nu=120 fixture) equals the dense autograd."""
+    inp = synth.make_inputs(*case)
+    want = dense.grad_dense(inp)
+    got = dense.grad_tuplewise(inp)
+    assert abs(got["e"] - want["e"]) <= 1e-13 * max(1.0, abs(want["e"]))
+    for k in ("t1", "t2", "vm", "voe", "vvvo"):
+        np.testing.assert_allclose(got[k], want[k], rtol=0, atol=1e-12 * np.abs(want[k]).max())
+
+
+@pytest.mark.parametrize("case", [(3, 5, 7), (4, 6, 11)])
+def test_grad_dense_pinned(case):
+    """grad_dense against what fixes it without autograd: its energy is the port's (pinned to
+    the reference), dE/dT1 = 2 v1^T exactly, and central differences of the dense energy
+    along a random direction of each array agree with grad . direction."""
+    inp = synth.make_inputs(*case)
+    g = dense.grad_dense(inp)
+    p = port.triples(inp)
+    assert abs(g["e"] - p["e"]) <= 1e-14 * max(1.0, abs(p["e"]))
+    np.testing.assert_allclose(g["t1"], 2.0 * p["v1"].T, rtol=0, atol=1e-15 * np.abs(p["v1"]).max())
+    rng = np.random.default_rng(3)
+    for k in ("t1", "t2", "vm", "voe", "vvvo"):
+        d = rng.standard_normal(inp[k].shape)
+        h = 1e-4 * np.sqrt(np.mean(inp[k] ** 2))
+        ep_ = dense.energies(dict(inp, **{k: np.asfortranarray(inp[k] + h * d)}))["e"]
+        em_ = dense.energies(dict(inp, **{k: np.asfortranarray(inp[k] - h * d)}))["e"]
+        fd = (ep_ - em_) / (2 * h)
+        an = float(np.sum(g[k] * d))
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3 * np.linalg.norm(g[k]) * np.linalg.norm(d)), (k, fd, an)
+
+
+def test_port_matches_reference_at_full_size():
+    """The C restatement against the reference's own per-tuple routines at no=24, nu=216
+    (tests/golden/golden_c4.npz): one distinct and one (i,i,j) tuple, E[T] to 1e-13 relative,
+    v1 columns and sampled W entries to 1e-13 of their max."""
+    import os
+    from golden.make_golden_c4 import NO, NU, SEED, occupied, tag
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_c4.npz"), allow_pickle=False) as f:
+        gold = {k: f[k] for k in f.files}
+    inp = synth.make_inputs(NO, NU, SEED)
+    from golden.make_golden import checksum as cs
+    np.testing.assert_allclose(cs(inp), gold["sum"], rtol=1e-13, atol=1e-14)
+    for t in [(0, 23, 22, 21), (2, 20, 7, 7)]:
+        kind, i, j, k = t
+        r = port.tuple_(inp, kind, i, j, k)
+        want = gold["etd_" + tag(t)][0]
+        assert abs(r["etd"] - want) <= 1e-13 * abs(want)
+        v1w = gold["v1_" + tag(t)]
+        got = r["v1"][:, list(occupied(t))]
+        np.testing.assert_allclose(got, v1w, rtol=0, atol=1e-13 * np.abs(v1w).max())
+        if kind == 0:
+            idx = gold["widx_" + tag(t)]
+            w = r["v3"][idx[:, 0], idx[:, 1], idx[:, 2]]
+            np.testing.assert_allclose(w, gold["w_" + tag(t)], rtol=0, atol=1e-13 * np.abs(w).max())

@@ -74,9 +74,39 @@ def rocpd_summary(paths: list) -> None:
         print()
 
 
-def cpu_baseline(inp_dev: dict, no: int, nu: int, budget_s: float, threads: int) -> dict | None:
-    """Reference per-tuple routines on a bounded sample of distinct tuples, extrapolated
-    to the whole tuple set by FLOP weight (a degenerate tuple is half a distinct one)."""
+def host_threads() -> tuple:
+    """(threads used, CPUs this process may run on): every CPU of the affinity mask, capped by
+    OMP_NUM_THREADS when it is set (the GPU box sets it to one GPU's share of its cores)."""
+    ncpu = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return (min(ncpu, int(cap)) if cap.isdigit() and int(cap) > 0 else ncpu), ncpu
+
+
+def sampled_tuples(no: int, n_dist: int, n_pairs: int) -> list:
+    """The first n_dist distinct tuples (DDCC_T_TASK order, src/triples-ijk/common_functions.f:62-86)
+    and the (i,j,j), (i,i,j) tuples of the first n_pairs pairs i>j, 1-based."""
+    out = []
+    for i in range(no):
+        for j in range(i):
+            for k in range(j):
+                if len(out) < n_dist:
+                    out.append((i + 1, j + 1, k + 1))
+    p = 0
+    for i in range(no):
+        for j in range(i):
+            if p < n_pairs:
+                out += [(i + 1, j + 1, j + 1), (i + 1, i + 1, j + 1)]
+            p += 1
+    return out
+
+
+def cpu_baseline(inp_dev: dict, ctx, no: int, nu: int, budget_s: float, threads: int, ncpu: int) -> dict | None:
+    """The reference's own compiled per-tuple routines (oracle/_ref: ijk_tuple + t1wt3_ijk for
+    distinct tuples, the ijj/iij formers + T3SQUA for degenerate ones, MKL DGEMM) on the host.
+    The whole tuple set runs when the calibrated estimate fits the budget; otherwise a prefix of
+    the distinct tuples and one of the degenerate pairs, each sized to its share of the FLOPs, are
+    timed and extrapolated per kind.  The sampled tuples' E[T] and v1 are compared with the GPU's
+    (ccx_energy_tuples over the same tuples)."""
     import numpy as np
 
     from oracle import ref
@@ -92,19 +122,39 @@ def cpu_baseline(inp_dev: dict, no: int, nu: int, budget_s: float, threads: int)
     del vv
     host["vvvo"] = None
     n_dist = no * (no - 1) * (no - 2) // 6
-    n_deg = no * (no - 1)
-    # calibrate on 2 tuples, then size the sample to the budget
-    cal = ref.triples(host, 0, min(2, n_dist), degenerate=False, threads=threads, packed=packed)
-    per = cal["seconds"] / max(1, min(2, n_dist))
-    sample = int(max(2, min(n_dist, budget_s / max(per, 1e-6))))
-    r = ref.triples(host, 0, sample, degenerate=False, threads=threads, packed=packed)
-    per = r["seconds"] / sample
-    full = per * (n_dist + 0.5 * n_deg)
+    n_pairs = no * (no - 1) // 2
+    # calibration: two distinct tuples, then one degenerate pair (two tuples)
+    cal_d = min(2, n_dist)
+    per_d = ref.triples(host, 0, cal_d, degenerate=False, threads=threads, packed=packed)["seconds"] / max(cal_d, 1)
+    per_p = ref.triples(host, 0, 0, deg_pairs=min(1, n_pairs), threads=threads, packed=packed)["seconds"]
+    full_est = per_d * n_dist + per_p * n_pairs
+    if full_est <= 1.5 * budget_s:
+        s_d, s_p = n_dist, n_pairs
+    else:
+        share_d = per_d * n_dist / full_est
+        s_d = int(max(min(2, n_dist), min(n_dist, budget_s * share_d / max(per_d, 1e-9))))
+        s_p = int(max(min(1, n_pairs), min(n_pairs, budget_s * (1 - share_d) / max(per_p, 1e-9))))
+    r_d = ref.triples(host, 0, s_d, degenerate=False, threads=threads, packed=packed)
+    r_p = ref.triples(host, 0, 0, deg_pairs=s_p, threads=threads, packed=packed)
+    per_d = r_d["seconds"] / max(s_d, 1)
+    per_p = r_p["seconds"] / max(s_p, 1)
+    full = per_d * n_dist + per_p * n_pairs
+    whole = s_d == n_dist and s_p == n_pairs
+    # parity of the sampled tuples: the reference's partial sums against the GPU's over the same set
+    gpu = ctx.energy_tuples(sampled_tuples(no, s_d, s_p))
+    etd_ref = r_d["etd"] + r_p["etd"]
+    v1_ref = r_d["v1"] + r_p["v1"]
+    rel_de = abs(gpu["etd"] - etd_ref) / abs(etd_ref)
+    rel_dv1 = float(np.abs(gpu["v1"] - v1_ref).max() / np.abs(v1_ref).max())
+    log(f"cpu_baseline sample: {s_d} distinct + {2 * s_p} degenerate tuples, reference {r_d['seconds'] + r_p['seconds']:.1f} s; "
+        f"GPU vs reference on them: E[T] rel {rel_de:.2e}, v1 rel {rel_dv1:.2e}")
     return {"value": full, "unit": "s", "cores": threads, "kind": "reference",
-            "sample": f"{sample} of {n_dist} distinct tuples timed ({r['seconds']:.1f} s in the reference's "
-                      f"ijk_tuple + t1wt3_ijk with MKL on {threads} threads), extrapolated by FLOP weight to "
-                      f"{n_dist} distinct + {n_deg} degenerate tuples",
-            "tuples_per_s": 1.0 / per}
+            "sample": ("the whole tuple set" if whole else
+                       f"{s_d} of {n_dist} distinct tuples ({r_d['seconds']:.1f} s) and {2 * s_p} of {2 * n_pairs} "
+                       f"degenerate tuples ({r_p['seconds']:.1f} s) timed in the reference's own routines (MKL, "
+                       f"{threads} threads), extrapolated per kind to all tuples"),
+            "host_cpus": ncpu, "tuples_per_s": (n_dist + 2 * n_pairs) / full,
+            "sample_parity": {"tuples": s_d + 2 * s_p, "etd_rel_diff": rel_de, "v1_rel_diff": rel_dv1}}
 
 
 def reduced_adjoint(ctx, g64: dict, form: str) -> dict:
@@ -126,7 +176,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every allowed CPU, capped by OMP_NUM_THREADS")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
@@ -261,10 +311,13 @@ def main() -> None:
             sa["ratio_to_forward"] = sa["ms"] / (kernel_s * 1e3)
             out["adjoint"][form] = sa
         del g
-    ctx.close()
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(inp, no, nu, args.cpu_budget, args.cpu_threads)
+        threads, ncpu = host_threads()
+        if args.cpu_threads > 0:
+            threads = args.cpu_threads
+        out["cpu_baseline"] = cpu_baseline(inp, ctx, no, nu, args.cpu_budget, threads, ncpu)
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 

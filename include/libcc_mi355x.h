@@ -44,10 +44,21 @@ extern "C" {
  * reductions are FP64 in every form */
 #define CCX_FORM_FP64 0    /* full FP64 (v_mfma_f64_16x16x4_f64)              */
 #define CCX_FORM_FP32 1    /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32) */
-#define CCX_FORM_BF16X3 2  /* split precision: each FP64 operand as hi + mid + lo bf16 slices (about
-                            * 26 bits of each element, under one power-of-two scale per slab), the 6
-                            * slice products with i+j <= 2 on v_mfma_f32_16x16x32_bf16 with FP32
-                            * accumulation; W, the epilogue and all reductions in FP64 */
+/* Split precision (Ozaki-style): each FP64 operand element is carried as several low-precision
+ * slices whose sum reproduces it; the slice products of levels i + j < slices run on the BF16 or
+ * FP8 matrix cores with FP32 accumulation, W is formed in FP64, and the denominators, the epilogue
+ * and every reduction stay FP64.
+ *   BF16X3: 3 bf16 slices per element, each the bf16 rounding of what the previous ones left (24
+ *           bits relative to the element itself), 6 products, one FP32 accumulator.
+ *   FP8X4:  4 fp8 e4m3 digit slices per 32-element row group under the group's own power-of-two
+ *           exponent (16 bits of the group maximum), 10 products on
+ *           v_mfma_scale_f32_16x16x128_f8f6f4 with exponent and slice weight as block scales.
+ *   BF16X4: 4 bf16 fixed-point digit slices under one power-of-two scale per slab (32 bits of the
+ *           slab maximum), 10 products; level 0 in its own FP32 accumulator (its products share
+ *           one grid and add exactly), the higher levels in a second one. */
+#define CCX_FORM_BF16X3 2
+#define CCX_FORM_FP8X4 3
+#define CCX_FORM_BF16X4 4
 
 const char *ccx_error_string(int status);
 int ccx_last_status(void);          /* status of the last legacy (void) call    */

@@ -381,12 +381,13 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
         if constexpr (FK != 0) {
             f4 acc[6][BPW];
             if constexpr (FK == 2) {
-                if (tkind == 2) item_w_split<M_IJJ>(g, lds, it, acc, wave, lane);
-                else if (tkind == 3) item_w_split<M_IIJ>(g, lds, it, acc, wave, lane);
-                else if (mult == 1) item_w_split<M_FULL>(g, lds, it, acc, wave, lane);
-                else if (mult == 6) item_w_split<M_AAA>(g, lds, it, acc, wave, lane);
-                else if (it.blk[0] == it.blk[1]) item_w_split<M_AAC>(g, lds, it, acc, wave, lane);
-                else item_w_split<M_ACC>(g, lds, it, acc, wave, lane);
+                f4 acc1[1][BPW];   // unused: the BF16 x 3 form has one accumulator
+                if (tkind == 2) item_w_split<M_IJJ, 2>(g, lds, it, acc, acc1, wave, lane);
+                else if (tkind == 3) item_w_split<M_IIJ, 2>(g, lds, it, acc, acc1, wave, lane);
+                else if (mult == 1) item_w_split<M_FULL, 2>(g, lds, it, acc, acc1, wave, lane);
+                else if (mult == 6) item_w_split<M_AAA, 2>(g, lds, it, acc, acc1, wave, lane);
+                else if (it.blk[0] == it.blk[1]) item_w_split<M_AAC, 2>(g, lds, it, acc, acc1, wave, lane);
+                else item_w_split<M_ACC, 2>(g, lds, it, acc, acc1, wave, lane);
             } else {
                 if (tkind == 2) item_w_f32v<M_IJJ>(g, lds, it, acc, wave, lane);
                 else if (tkind == 3) item_w_f32v<M_IIJ>(g, lds, it, acc, wave, lane);
@@ -895,6 +896,7 @@ int ccx_hip_adj_items(const void *axr, const void *axc, const void *bx, const do
     g.axr_s = sp ? w32[0] : nullptr;
     g.axc_s = sp ? w32[1] : nullptr;
     g.bx_s = sp ? w32[2] : nullptr;
+    g.axr_e = g.axc_e = g.bx_e = nullptr;
     g.kps = kps;
     g.wscale = sp ? wscale : 1.0;
     ga.t1 = t1; ga.wbar = wbar; ga.slots = slots; ga.wstage = wstage; ga.t0 = t0; ga.tcount = tcount;

@@ -22,11 +22,18 @@ int ccx_hip_nonfinite(const double *const *p, const size_t *n, int narr, unsigne
                       void *stream);
 int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *vm, const double *voe, void *axr,
                  void *axc, void *bx, double *voep, int no, int nu, int nup, int kp, void *stream);
+int ccx_hip_pack_ax_one(const double *vvvo_x, const double *t2_x, double *axr_x, double *axc_x, int no, int nu,
+                        int nup, int kp, void *stream);
+int ccx_hip_pack_pair(const double *t2_yz, const double *vm_zy, const double *voe_yz, double *bx_yz,
+                      double *voep_yz, int no, int nu, int nup, int kp, void *stream);
+int ccx_hip_ve_to_vvvo(const double *ve, double *vvvo_x, int nu, void *stream);
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
                     int ntup, int nblk, int grid, const void *const *split, int kps, double wscale, void *stream);
-int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, void *axr_s, void *axc_s, void *bx_s,
+int ccx_hip_form_kpad(int form);
+int ccx_hip_split_elem_bytes(int form);
+int ccx_hip_pack_split(int form, const double *axr, const double *axc, const double *bx, void *const *sl,
                        unsigned long long *dmax, int no, int nb, int kp, int kps, double *wscale, void *stream);
 int ccx_hip_pack_f32v(const double *axr, const double *axc, const double *bx, float *axr32, float *axc32,
                       float *bx32, int no, int nb, int kp, int kpf, void *stream);

@@ -23,11 +23,13 @@ struct ccx_ctx {
     float *d_axr32, *d_axc32, *d_bx32; /* FP32 form slabs, converted on first use after a load */
     int64_t kpf;                        /* K padded to 48 for the FP32 form */
     int f32_ready;
-    void *d_axr_s, *d_axc_s, *d_bx_s;  /* split BF16 form: 3 bf16 slices per element, built on first use */
+    void *d_axr_s, *d_axc_s, *d_bx_s;  /* split forms: slices of each element, built on first use */
+    int *d_axr_e, *d_axc_e, *d_bx_e;   /* FP8 form: per-lane block exponent words */
+    size_t split_ax_bytes, split_bx_bytes;   /* allocated sizes of the slice slabs */
     unsigned long long *d_max;
-    int64_t kps;                        /* K padded to 64 for the split form */
+    int64_t kps;                        /* K padding of the split form whose slices are built */
     double split_scale;
-    int split_ready;
+    int split_ready, split_form;
     int *d_tuples, *d_blocks;
     unsigned int *d_counter;
     unsigned int *d_flag;               /* NaN/Inf flag of ccx_load_device */
@@ -116,6 +118,11 @@ static void free_dev(ccx_ctx *c)
     ccx_hip_free(c->d_axc_s);
     ccx_hip_free(c->d_bx_s);
     ccx_hip_free(c->d_max);
+    ccx_hip_free(c->d_axr_e);
+    ccx_hip_free(c->d_axc_e);
+    ccx_hip_free(c->d_bx_e);
+    c->d_axr_e = c->d_axc_e = c->d_bx_e = NULL;
+    c->split_ax_bytes = c->split_bx_bytes = 0;
     c->d_tuples = c->d_blocks = NULL;
     c->d_counter = NULL;
     c->d_axr32 = c->d_axc32 = c->d_bx32 = NULL;
@@ -140,7 +147,6 @@ int ccx_create(int64_t no, int64_t nu, ccx_ctx **out)
     c->nup = (nu + TB - 1) / TB * TB;
     c->nb = c->nup / TB;
     c->kp = (nu + no + 3) / 4 * 4;
-    c->kps = (nu + no + 63) / 64 * 64;
     c->kpf = (nu + no + 47) / 48 * 48;
     c->ntup = build_tuples(no, NULL);
     c->nblk = build_blocks(c->nb, NULL);
@@ -336,28 +342,54 @@ static int ensure_f32(ccx_ctx *c)
     return CCX_OK;
 }
 
-/* split BF16 slabs (3 bf16 slice planes of each FP64 slab), allocated on first use */
-static int ensure_split(ccx_ctx *c)
+/* the slices of a split form (CCX_FORM_BF16X3, _FP8X4, _BF16X4) from the FP64 slabs, built on the
+ * first call in that form after a load; the slice slabs are reallocated when a form needs more
+ * room than the last one (the FP8 form's exponent words are allocated with its first use) */
+static int ensure_split(ccx_ctx *c, int form)
 {
-    if (c->split_ready) return CCX_OK;
-    const int64_t ax_slice = c->no * c->nb * c->nb * 256 * c->kps, bx_slice = c->no * c->no * c->nb * 16 * c->kps;
-    if (!c->d_axr_s) {
-        if (ccx_hip_malloc(&c->d_axr_s, 2 * 3 * (size_t)ax_slice) != 0 ||
-            ccx_hip_malloc(&c->d_axc_s, 2 * 3 * (size_t)ax_slice) != 0 ||
-            ccx_hip_malloc(&c->d_bx_s, 2 * 3 * (size_t)bx_slice) != 0 ||
+    if (c->split_ready && c->split_form == form) return CCX_OK;
+    const int64_t pad = ccx_hip_form_kpad(form), eb = ccx_hip_split_elem_bytes(form);
+    const int64_t kps = (c->nu + c->no + pad - 1) / pad * pad;
+    const size_t ax_bytes = (size_t)(c->no * c->nb * c->nb * 256 * kps * eb);
+    const size_t bx_bytes = (size_t)(c->no * c->no * c->nb * 16 * kps * eb);
+    c->split_ready = 0;
+    if (!c->d_axr_s || ax_bytes > c->split_ax_bytes || bx_bytes > c->split_bx_bytes) {
+        ccx_hip_free(c->d_axr_s); ccx_hip_free(c->d_axc_s); ccx_hip_free(c->d_bx_s); ccx_hip_free(c->d_max);
+        c->d_axr_s = c->d_axc_s = c->d_bx_s = NULL;
+        c->d_max = NULL;
+        c->split_ax_bytes = c->split_bx_bytes = 0;
+        if (ccx_hip_malloc(&c->d_axr_s, ax_bytes) != 0 || ccx_hip_malloc(&c->d_axc_s, ax_bytes) != 0 ||
+            ccx_hip_malloc(&c->d_bx_s, bx_bytes) != 0 ||
             ccx_hip_malloc((void **)&c->d_max, 2 * sizeof(unsigned long long)) != 0) {
             ccx_hip_free(c->d_axr_s); ccx_hip_free(c->d_axc_s); ccx_hip_free(c->d_bx_s); ccx_hip_free(c->d_max);
             c->d_axr_s = c->d_axc_s = c->d_bx_s = NULL;
             c->d_max = NULL;
             return CCX_ERR_ALLOC;
         }
+        c->split_ax_bytes = ax_bytes;
+        c->split_bx_bytes = bx_bytes;
     }
-    if (ccx_hip_pack_split(c->d_axr, c->d_axc, c->d_bx, c->d_axr_s, c->d_axc_s, c->d_bx_s, c->d_max, (int)c->no,
-                           (int)c->nb, (int)c->kp, (int)c->kps, &c->split_scale, c->stream) != 0)
+    if (form == CCX_FORM_FP8X4 && !c->d_axr_e) {   /* one word per 32 elements */
+        const size_t ax_w = (size_t)(c->no * c->nb * c->nb * 256 * kps / 32), bx_w = (size_t)(c->no * c->no * c->nb * 16 * kps / 32);
+        if (ccx_hip_malloc((void **)&c->d_axr_e, sizeof(int) * ax_w) != 0 ||
+            ccx_hip_malloc((void **)&c->d_axc_e, sizeof(int) * ax_w) != 0 ||
+            ccx_hip_malloc((void **)&c->d_bx_e, sizeof(int) * bx_w) != 0) {
+            ccx_hip_free(c->d_axr_e); ccx_hip_free(c->d_axc_e); ccx_hip_free(c->d_bx_e);
+            c->d_axr_e = c->d_axc_e = c->d_bx_e = NULL;
+            return CCX_ERR_ALLOC;
+        }
+    }
+    void *sl[6] = {c->d_axr_s, c->d_axc_s, c->d_bx_s, c->d_axr_e, c->d_axc_e, c->d_bx_e};
+    if (ccx_hip_pack_split(form, c->d_axr, c->d_axc, c->d_bx, sl, c->d_max, (int)c->no, (int)c->nb, (int)c->kp,
+                           (int)kps, &c->split_scale, c->stream) != 0)
         return CCX_ERR_DEVICE;
+    c->kps = kps;
+    c->split_form = form;
     c->split_ready = 1;
     return CCX_OK;
 }
+
+static int is_split_form(int form) { return form == CCX_FORM_BF16X3 || form == CCX_FORM_FP8X4 || form == CCX_FORM_BF16X4; }
 
 /* One persistent launch over the tuple table d_tup (ntup entries of 4 ints) and the
  * deterministic reductions; energies to ets and etd, v1 (nu,no) to v1 when non-null. */
@@ -370,11 +402,11 @@ static int energy_run(ccx_ctx *c, int form, const int *d_tup, int64_t ntup, doub
         int rc = ensure_f32(c);
         if (rc != CCX_OK) return rc;
     }
-    if (form == CCX_FORM_BF16X3) {
-        int rc = ensure_split(c);
+    if (is_split_form(form)) {
+        int rc = ensure_split(c, form);
         if (rc != CCX_OK) return rc;
     }
-    const void *split[3] = {c->d_axr_s, c->d_axc_s, c->d_bx_s};
+    const void *split[6] = {c->d_axr_s, c->d_axc_s, c->d_bx_s, c->d_axr_e, c->d_axc_e, c->d_bx_e};
     const void *axr = form == CCX_FORM_FP32 ? (const void *)c->d_axr32 : (const void *)c->d_axr;
     const void *axc = form == CCX_FORM_FP32 ? (const void *)c->d_axc32 : (const void *)c->d_axc;
     const void *bx = form == CCX_FORM_FP32 ? (const void *)c->d_bx32 : (const void *)c->d_bx;
@@ -410,7 +442,7 @@ static int energy_run(ccx_ctx *c, int form, const int *d_tup, int64_t ntup, doub
 
 static int form_known(int form)
 {
-    return form == CCX_FORM_FP64 || form == CCX_FORM_FP32 || form == CCX_FORM_BF16X3;
+    return form == CCX_FORM_FP64 || form == CCX_FORM_FP32 || is_split_form(form);
 }
 
 int ccx_energy(ccx_ctx *c, int form, double *ets, double *etd, double *v1, int v1_on_device)
@@ -657,7 +689,7 @@ int ccx_gradient_form(ccx_ctx *c, int form, double *ets, double *etd, double *dt
     if (!c->loaded) return CCX_ERR_STATE;
     if (form != CCX_FORM_FP64 && form != CCX_FORM_FP32 && form != CCX_FORM_BF16X3) return CCX_ERR_FORM;
     if (form == CCX_FORM_BF16X3) {
-        int rs = ensure_split(c);
+        int rs = ensure_split(c, form);
         if (rs != CCX_OK) return rs;
     }
     if (form == CCX_FORM_FP32) {
@@ -932,52 +964,233 @@ void cc_triples_(double *eh, double *ep, double *v1, double *t1, double *t2, dou
     legacy_triples(g_d_vvvo, eh, ep, v1, t1, t2, vm, voe, NULL, NULL);
 }
 
-/* W_ijk (0-based tuple) into v3 (nu^3) from the slices the reference's ijk_tuple receives,
- * on the GPU: full-size arrays hold only those slices (the rest is zero), then the
- * single-tuple W of the persistent kernel; W(a,a,a) = 0 as zerot3 leaves it. */
-static int tuple_w_host(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk, const double *t2_i,
-                        const double *t2_j, const double *t2_k, const double *vm_ij, const double *vm_ji,
-                        const double *vm_ik, const double *vm_ki, const double *vm_jk, const double *vm_kj,
-                        const double *ve_i, const double *ve_j, const double *ve_k, double *v3)
+/* ------------------------------------------------------------------ per-tuple engine */
+
+/* The per-tuple entries (ijk_tuple_, t1wt3_ijk_, ddcc_t_*_acc_, ddcc_t_ijk_big_, the ijk_gpu_*
+ * trio) drive the reference's tuple loop one call at a time.  They share one persistent device
+ * workspace, kept across calls while (no, nu) stays the same: operand panels for three occupied
+ * slots (the tuple's indices), BX panels and padded VOE blocks for the 3 x 3 ordered slot pairs,
+ * and the partial buffers of one tuple's work items.  A call uploads only the slices it is given
+ * or that its tuple reads (VE_x or the packed VVVO patch, T2(:,:,:,x), VM(:,:,z,y), VOE(:,:,y,z)),
+ * packs those panels, and runs the persistent kernel over its one tuple with the tuple's indices
+ * renamed to slots 0..2 (the kernel addresses AX by slot, BX and VOE by slot pair). */
+#define TE_SLOTS 3
+static struct {
+    int64_t no, nu, nup, nb, kp, nblk;
+    double *axr, *axc, *bx, *voep, *eh, *ep, *t1, *epart, *v1part, *v1t, *v1, *out;
+    double *vvvox, *t2x, *vmb, *voeb, *in, *x3, *v1p;
+    int *tup, *blocks;
+    unsigned int *counter;
+    double *h_ep;
+    int ready;
+} g_te;
+
+static void te_free(void)
 {
-    const int64_t nu2 = nu * nu;
-    if (no < 1 || nu < 1) return CCX_ERR_STATE;
-    if (ii < 0 || jj < 0 || kk < 0 || ii >= no || jj >= no || kk >= no) return CCX_ERR_ARG;
-    const size_t n_t2 = (size_t)(nu2 * no * no), n_vm = (size_t)(no * nu * no * no);
-    double *t2 = (double *)calloc(n_t2, sizeof(double)), *vm = (double *)calloc(n_vm, sizeof(double));
-    double *voe = (double *)calloc(n_t2, sizeof(double));
-    double *vvvo = (double *)calloc((size_t)(nu2 * nu * no), sizeof(double));
-    double *eh = (double *)calloc((size_t)no, sizeof(double)), *ep = (double *)calloc((size_t)nu, sizeof(double));
-    double *t1 = (double *)calloc((size_t)(no * nu), sizeof(double));
-    ccx_ctx *c = NULL;
+    double *d[] = {g_te.axr, g_te.axc, g_te.bx, g_te.voep, g_te.eh, g_te.ep, g_te.t1, g_te.epart, g_te.v1part,
+                   g_te.v1t, g_te.v1, g_te.out, g_te.vvvox, g_te.t2x, g_te.vmb, g_te.voeb, g_te.in, g_te.x3,
+                   g_te.v1p};
+    for (size_t q = 0; q < sizeof(d) / sizeof(d[0]); ++q) ccx_hip_free(d[q]);
+    ccx_hip_free(g_te.tup);
+    ccx_hip_free(g_te.blocks);
+    ccx_hip_free(g_te.counter);
+    free(g_te.h_ep);
+    memset(&g_te, 0, sizeof(g_te));
+}
+
+/* the workspace for (no, nu), allocated on first use and whenever the dimensions change */
+static int te_ensure(int64_t no, int64_t nu)
+{
+    if (no < 1 || nu < 1 || no > 2048 || nu > 4096) return CCX_ERR_ARG;
+    if (g_te.ready && g_te.no == no && g_te.nu == nu) return CCX_OK;
+    te_free();
+    int ndev = 0;
+    if (ccx_hip_device_count(&ndev) != 0 || ndev < 1) return CCX_ERR_DEVICE;
     int rc = CCX_OK;
-    if (!t2 || !vm || !voe || !vvvo || !eh || !ep || !t1) { rc = CCX_ERR_ALLOC; goto done; }
-    {
-        const int64_t xs[3] = {ii, jj, kk};
-        const double *t2s[3] = {t2_i, t2_j, t2_k};
-        const double *ves[3] = {ve_i, ve_j, ve_k};
-        for (int p = 0; p < 3; ++p) {
-            memcpy(t2 + (size_t)(nu2 * no * xs[p]), t2s[p], sizeof(double) * (size_t)(nu2 * no));
-            /* VVVO(d,q,p,x) = VE_x(d,p,q) */
-            double *dst = vvvo + (size_t)(nu2 * nu * xs[p]);
-            for (int64_t q = 0; q < nu; ++q)
-                for (int64_t pp = 0; pp < nu; ++pp)
-                    for (int64_t d = 0; d < nu; ++d) dst[d + nu * (q + nu * pp)] = ves[p][d + nu * (pp + nu * q)];
-        }
-        const double *vms[6] = {vm_ij, vm_ji, vm_ik, vm_ki, vm_jk, vm_kj};
-        const int64_t vx[6] = {ii, jj, ii, kk, jj, kk}, vy[6] = {jj, ii, kk, ii, kk, jj};
-        for (int p = 0; p < 6; ++p)
-            memcpy(vm + (size_t)(no * nu * (vx[p] + no * vy[p])), vms[p], sizeof(double) * (size_t)(no * nu));
+    int *h_blk = NULL;
+    const int64_t nup = (nu + TB - 1) / TB * TB, nb = nup / TB, kp = (nu + no + 3) / 4 * 4;
+    const int64_t nblk = build_blocks(nb, NULL), S = TE_SLOTS, nu3 = nu * nu * nu;
+    g_te.no = no; g_te.nu = nu; g_te.nup = nup; g_te.nb = nb; g_te.kp = kp; g_te.nblk = nblk;
+    DALLOC(g_te.axr, S * nb * nb * kp * 256);
+    DALLOC(g_te.axc, S * nb * nb * kp * 256);
+    DALLOC(g_te.bx, S * S * nb * kp * 16);
+    DALLOC(g_te.voep, S * S * nup * nup);
+    DALLOC(g_te.eh, S);
+    DALLOC(g_te.ep, nup);
+    DALLOC(g_te.t1, S * nu);
+    DALLOC(g_te.epart, nblk);
+    DALLOC(g_te.v1part, nblk * 144);
+    DALLOC(g_te.v1t, 3 * nu);
+    DALLOC(g_te.v1, S * nu);
+    DALLOC(g_te.out, 4);
+    DALLOC(g_te.vvvox, S * nu3);
+    DALLOC(g_te.t2x, S * nu * nu * no);
+    DALLOC(g_te.vmb, S * S * no * nu);
+    DALLOC(g_te.voeb, S * S * nu * nu);
+    DALLOC(g_te.in, nu3);
+    DALLOC(g_te.x3, nu);
+    DALLOC(g_te.v1p, 3 * nu);
+    DALLOC(g_te.tup, 4);
+    DALLOC(g_te.blocks, 4 * nblk);
+    DALLOC(g_te.counter, 8);
+    g_te.h_ep = (double *)malloc(sizeof(double) * (size_t)nup);
+    h_blk = (int *)malloc(sizeof(int) * 4 * (size_t)nblk);
+    if (!g_te.h_ep || !h_blk) { rc = CCX_ERR_ALLOC; goto fail; }
+    build_blocks(nb, h_blk);
+    /* everything a call does not overwrite stays zero and finite */
+    if (ccx_hip_zero(g_te.voep, sizeof(double) * (size_t)(S * S * nup * nup), NULL) != 0 ||
+        ccx_hip_zero(g_te.eh, sizeof(double) * (size_t)S, NULL) != 0 ||
+        ccx_hip_zero(g_te.t1, sizeof(double) * (size_t)(S * nu), NULL) != 0 ||
+        ccx_hip_memcpy_h2d(g_te.blocks, h_blk, sizeof(int) * 4 * (size_t)nblk, NULL) != 0) {
+        rc = CCX_ERR_DEVICE;
+        goto fail;
     }
-    rc = ccx_create(no, nu, &c);
-    if (rc == CCX_OK) rc = ccx_load_host(c, eh, ep, t1, t2, vm, voe, vvvo, 0);
-    if (rc == CCX_OK) rc = ccx_tuple_w(c, ii + 1, jj + 1, kk + 1, v3);
-    if (rc == CCX_OK)
-        for (int64_t a = 0; a < nu; ++a) v3[a + nu * (a + nu * a)] = 0.0; /* zerot3 */
-done:
-    if (c) ccx_destroy(c);
-    free(t2); free(vm); free(voe); free(vvvo); free(eh); free(ep); free(t1);
+    for (int64_t a = 0; a < nup; ++a) g_te.h_ep[a] = 1.0e3;
+    if (ccx_hip_memcpy_h2d(g_te.ep, g_te.h_ep, sizeof(double) * (size_t)nup, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    free(h_blk);
+    g_te.ready = 1;
+    return CCX_OK;
+fail:
+    free(h_blk);
+    te_free();
     return rc;
+}
+
+/* One tuple through the workspace.  x[p]: the occupied index (0-based) at tuple position p; kind
+ * 1 (i,j,k), 2 (i,j,j), 3 (i,i,j).  vvvo_src[p]: the cube of position p's index, as VE_x(d,p,q)
+ * = VVVO(d,q,p,x) (src_kind 0) or as the distributed packed patch, nutr x nu (src_kind 1);
+ * NULL for a position that repeats an earlier index.  t2x[p] = T2(:,:,:,x[p]) (likewise NULL for
+ * a repeat).  vm(y,z) returns VM(:,:,x_y,x_z) for tuple positions y,z, voe(y,z) VOE(:,:,x_y,x_z);
+ * with voe == NULL no energy is formed.  Energy weights are those of the full set (1, 1/2).
+ * etd, v1 (nu,no, +=, columns x) and w (nu^3, W_ijk with W(a,a,a) = 0) are each optional. */
+typedef const double *(*te_block_fn)(const void *ud, int y, int z);
+static int te_tuple(int64_t no, int64_t nu, const int64_t x[3], int kind, const double *const vvvo_src[3],
+                    int src_kind, const double *const t2x[3], te_block_fn vm, te_block_fn voe, const void *ud,
+                    const double *eh, const double *ep, double *etd, double *v1, double *w)
+{
+    int rc = te_ensure(no, nu);
+    if (rc != CCX_OK) return rc;
+    for (int p = 0; p < 3; ++p)
+        if (x[p] < 0 || x[p] >= no) return CCX_ERR_ARG;
+    const int64_t nu2 = nu * nu, nu3 = nu2 * nu, nup = g_te.nup, kp = g_te.kp, nb = g_te.nb;
+    const int64_t nutr = nu * (nu + 1) / 2;
+    /* slots: position p -> slot sl[p]; the first position holding an index owns its slot */
+    int sl[3], nslot = 0;
+    for (int p = 0; p < 3; ++p) {
+        sl[p] = -1;
+        for (int q = 0; q < p; ++q)
+            if (x[q] == x[p]) sl[p] = sl[q];
+        if (sl[p] < 0) sl[p] = nslot++;
+    }
+    const int64_t panel = nb * nb * kp * 256;
+    double h_eh[TE_SLOTS] = {0.0, 0.0, 0.0};
+    if (eh)
+        for (int p = 0; p < 3; ++p) h_eh[sl[p]] = eh[x[p]];
+    for (int p = 0; p < 3; ++p) {
+        int first = 1;
+        for (int q = 0; q < p; ++q)
+            if (x[q] == x[p]) first = 0;
+        if (!first) continue;
+        if (!vvvo_src[p] || !t2x[p]) return CCX_ERR_ARG;
+        double *vx = g_te.vvvox + sl[p] * nu3;
+        if (src_kind == 0) {
+            if (ccx_hip_memcpy_h2d(g_te.in, vvvo_src[p], sizeof(double) * (size_t)nu3, NULL) != 0 ||
+                ccx_hip_ve_to_vvvo(g_te.in, vx, (int)nu, NULL) != 0)
+                return CCX_ERR_DEVICE;
+        } else {
+            if (ccx_hip_memcpy_h2d(g_te.in, vvvo_src[p], sizeof(double) * (size_t)(nutr * nu), NULL) != 0 ||
+                ccx_hip_unpack_vvvo(g_te.in, vx, 1, (int)nu, NULL) != 0)
+                return CCX_ERR_DEVICE;
+        }
+        double *tx = g_te.t2x + sl[p] * nu2 * no;
+        if (ccx_hip_memcpy_h2d(tx, t2x[p], sizeof(double) * (size_t)(nu2 * no), NULL) != 0 ||
+            ccx_hip_pack_ax_one(vx, tx, g_te.axr + sl[p] * panel, g_te.axc + sl[p] * panel, (int)no, (int)nu,
+                                (int)nup, (int)kp, NULL) != 0)
+            return CCX_ERR_DEVICE;
+    }
+    /* BX panels and VOE blocks of the ordered position pairs (y,z), y != z, at slot pair (sy,sz) */
+    for (int y = 0; y < 3; ++y)
+        for (int z = 0; z < 3; ++z) {
+            if (y == z) continue;
+            const int sy = sl[y], sz = sl[z], sp = sy * TE_SLOTS + sz;
+            const double *vmzy = vm(ud, z, y), *voeyz = voe ? voe(ud, y, z) : NULL;
+            if (!vmzy || (voe && !voeyz)) return CCX_ERR_ARG;
+            double *vmd = g_te.vmb + sp * no * nu, *voed = g_te.voeb + sp * nu2;
+            if (ccx_hip_memcpy_h2d(vmd, vmzy, sizeof(double) * (size_t)(no * nu), NULL) != 0 ||
+                (voeyz && ccx_hip_memcpy_h2d(voed, voeyz, sizeof(double) * (size_t)nu2, NULL) != 0))
+                return CCX_ERR_DEVICE;
+            /* T2(:,:,x_y,x_z) sits in the T2(:,:,:,x_z) slab of slot sz */
+            const double *t2yz = g_te.t2x + sz * nu2 * no + x[y] * nu2;
+            if (ccx_hip_pack_pair(t2yz, vmd, voeyz ? voed : NULL, g_te.bx + sp * nb * kp * 16,
+                                  g_te.voep + sp * nup * nup, (int)no, (int)nu, (int)nup, (int)kp, NULL) != 0)
+                return CCX_ERR_DEVICE;
+        }
+    if (ep) {
+        memcpy(g_te.h_ep, ep, sizeof(double) * (size_t)nu);
+        if (ccx_hip_memcpy_h2d(g_te.ep, g_te.h_ep, sizeof(double) * (size_t)nup, NULL) != 0) return CCX_ERR_DEVICE;
+    }
+    const int tup[4] = {sl[0], sl[1], sl[2], kind};
+    if (ccx_hip_memcpy_h2d(g_te.eh, h_eh, sizeof(h_eh), NULL) != 0 ||
+        ccx_hip_memcpy_h2d(g_te.tup, tup, sizeof(tup), NULL) != 0)
+        return CCX_ERR_DEVICE;
+    int cus = 256;
+    if (ccx_hip_cu_count(&cus) != 0 || cus < 1) cus = 256;
+    const int grid = (int)(g_te.nblk < cus ? g_te.nblk : cus);
+    if (ccx_hip_triples(0, g_te.axr, g_te.axc, g_te.bx, g_te.voep, g_te.eh, g_te.ep, g_te.tup, g_te.blocks,
+                        g_te.epart, g_te.v1part, w ? g_te.in : NULL, g_te.counter, TE_SLOTS, (int)nu, (int)nup,
+                        (int)kp, 1, (int)g_te.nblk, grid, NULL, 0, 1.0, NULL) != 0)
+        return CCX_ERR_DEVICE;
+    if (w && ccx_hip_memcpy_d2h(w, g_te.in, sizeof(double) * (size_t)nu3, NULL) != 0) return CCX_ERR_DEVICE;
+    if (w)
+        for (int64_t a = 0; a < nu; ++a) w[a + nu * (a + nu * a)] = 0.0; /* zerot3 */
+    if (voe && (etd || v1)) {
+        double out[2];
+        double *hv = (double *)malloc(sizeof(double) * (size_t)(TE_SLOTS * nu));
+        if (!hv) return CCX_ERR_ALLOC;
+        if (ccx_hip_finalize(g_te.epart, g_te.v1part, g_te.tup, g_te.blocks, g_te.t1, g_te.v1t, g_te.v1, g_te.out,
+                             TE_SLOTS, (int)nu, 1, (int)g_te.nblk, NULL) != 0 ||
+            ccx_hip_memcpy_d2h(out, g_te.out, sizeof(out), NULL) != 0 ||
+            ccx_hip_memcpy_d2h(hv, g_te.v1, sizeof(double) * (size_t)(TE_SLOTS * nu), NULL) != 0) {
+            free(hv);
+            return CCX_ERR_DEVICE;
+        }
+        if (etd) *etd += out[0];
+        if (v1)   /* slot column s -> occupied column of the first position holding it */
+            for (int s2 = 0; s2 < nslot; ++s2) {
+                int64_t col = -1;
+                for (int p = 0; p < 3 && col < 0; ++p)
+                    if (sl[p] == s2) col = x[p];
+                for (int64_t a = 0; a < nu; ++a) v1[a + nu * col] += hv[a + nu * s2];
+            }
+        free(hv);
+    }
+    return CCX_OK;
+}
+
+/* block lookups for the two argument styles: full VM / VOE arrays, or the six pair slices */
+struct te_full { const double *vm, *voe; int64_t no, nu; int64_t x[3]; };
+static const double *te_full_vm(const void *ud, int y, int z)
+{
+    const struct te_full *f = (const struct te_full *)ud;   /* VM(:,:,x_y,x_z) */
+    return f->vm + f->no * f->nu * (f->x[y] + f->no * f->x[z]);
+}
+static const double *te_full_voe(const void *ud, int y, int z)
+{
+    const struct te_full *f = (const struct te_full *)ud;   /* VOE(:,:,x_y,x_z) */
+    return f->voe + f->nu * f->nu * (f->x[y] + f->no * f->x[z]);
+}
+struct te_pairs { const double *blk[3][3]; };               /* blk[y][z] = (:,:,x_y,x_z) */
+static const double *te_pair_blk(const void *ud, int y, int z) { return ((const struct te_pairs *)ud)->blk[y][z]; }
+
+/* one tuple from the full arrays the driver passes: v1 (nu,no) +=, etd +=, W into v3 when non-null */
+static int te_full_tuple(int64_t no, int64_t nu, const int64_t x[3], int kind, const double *const src[3], int src_kind,
+                         const double *t2, const double *vm, const double *voe, const double *eh, const double *ep,
+                         double *etd, double *v1, double *v3)
+{
+    struct te_full f = {vm, voe, no, nu, {x[0], x[1], x[2]}};
+    const double *t2x[3];
+    for (int p = 0; p < 3; ++p) t2x[p] = t2 + nu * nu * no * x[p];
+    return te_tuple(no, nu, x, kind, src, src_kind, t2x, te_full_vm, te_full_voe, &f, eh, ep, etd, v1, v3);
 }
 
 void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const double *t2_j, const double *t2_k,
@@ -987,8 +1200,12 @@ void ijk_tuple_(int64_t *i, int64_t *j, int64_t *k, const double *t2_i, const do
 {
     if (!i || !j || !k || !t2_i || !t2_j || !t2_k || !vm_ij || !vm_ji || !vm_ik || !vm_ki || !vm_jk || !vm_kj ||
         !ve_i || !ve_j || !ve_k || !v3) { g_status = CCX_ERR_ARG; return; }
-    g_status = tuple_w_host(g_no, g_nu, *i - 1, *j - 1, *k - 1, t2_i, t2_j, t2_k, vm_ij, vm_ji, vm_ik, vm_ki, vm_jk,
-                            vm_kj, ve_i, ve_j, ve_k, v3);
+    if (g_no < 1 || g_nu < 1) { g_status = CCX_ERR_STATE; return; }
+    const int64_t x[3] = {*i - 1, *j - 1, *k - 1};
+    if (x[0] == x[1] || x[1] == x[2] || x[0] == x[2]) { g_status = CCX_ERR_ARG; return; }
+    struct te_pairs pr = {{{NULL, vm_ij, vm_ik}, {vm_ji, NULL, vm_jk}, {vm_ki, vm_kj, NULL}}};
+    const double *src[3] = {ve_i, ve_j, ve_k}, *t2x[3] = {t2_i, t2_j, t2_k};
+    g_status = te_tuple(g_no, g_nu, x, 1, src, 0, t2x, te_pair_blk, NULL, &pr, NULL, NULL, NULL, NULL, v3);
 }
 
 /* E[T] and v1 of one tuple (0-based i,j,k) from its W cube on the GPU (t1wt3_kernel):
@@ -997,25 +1214,20 @@ static int t1wt3_host(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk
                       const double *voe6[6], double *v1, const double *eh, const double *ep, double efac,
                       double vfac)
 {
-    if (no < 1 || nu < 1) return CCX_ERR_STATE;
     if (ii < 0 || jj < 0 || kk < 0 || ii >= no || jj >= no || kk >= no) return CCX_ERR_ARG;
+    int rc = te_ensure(no, nu);
+    if (rc != CCX_OK) return rc;
     const size_t nu2 = (size_t)(nu * nu), nu3 = nu2 * (size_t)nu;
-    int rc = CCX_OK;
-    double *d_w = NULL, *d_voe = NULL, *d_ep = NULL, *d_x3 = NULL, *d_v1p = NULL;
     double *x3 = (double *)malloc(sizeof(double) * (size_t)nu), *v1p = (double *)malloc(sizeof(double) * 3 * (size_t)nu);
-    if (!x3 || !v1p) { rc = CCX_ERR_ALLOC; goto fail; }
-    DALLOC(d_w, nu3);
-    DALLOC(d_voe, 6 * nu2);
-    DALLOC(d_ep, nu);
-    DALLOC(d_x3, nu);
-    DALLOC(d_v1p, 3 * nu);
-    if (ccx_hip_memcpy_h2d(d_w, v3, sizeof(double) * nu3, NULL) != 0 ||
-        ccx_hip_memcpy_h2d(d_ep, ep, sizeof(double) * (size_t)nu, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+    if (!x3 || !v1p) { rc = CCX_ERR_ALLOC; goto done; }
+    /* W in the staging cube, the six VOE slices in the first six pair blocks, eps_virtual in ep */
+    if (ccx_hip_memcpy_h2d(g_te.in, v3, sizeof(double) * nu3, NULL) != 0 ||
+        ccx_hip_memcpy_h2d(g_te.ep, ep, sizeof(double) * (size_t)nu, NULL) != 0) { rc = CCX_ERR_DEVICE; goto done; }
     for (int q = 0; q < 6; ++q)
-        if (ccx_hip_memcpy_h2d(d_voe + q * nu2, voe6[q], sizeof(double) * nu2, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
-    if (ccx_hip_t1wt3(d_w, d_voe, eh[ii] + eh[jj] + eh[kk], d_ep, (int)nu, d_x3, d_v1p, NULL) != 0 ||
-        ccx_hip_memcpy_d2h(x3, d_x3, sizeof(double) * (size_t)nu, NULL) != 0 ||
-        ccx_hip_memcpy_d2h(v1p, d_v1p, sizeof(double) * 3 * (size_t)nu, NULL) != 0) { rc = CCX_ERR_DEVICE; goto fail; }
+        if (ccx_hip_memcpy_h2d(g_te.voeb + q * nu2, voe6[q], sizeof(double) * nu2, NULL) != 0) { rc = CCX_ERR_DEVICE; goto done; }
+    if (ccx_hip_t1wt3(g_te.in, g_te.voeb, eh[ii] + eh[jj] + eh[kk], g_te.ep, (int)nu, g_te.x3, g_te.v1p, NULL) != 0 ||
+        ccx_hip_memcpy_d2h(x3, g_te.x3, sizeof(double) * (size_t)nu, NULL) != 0 ||
+        ccx_hip_memcpy_d2h(v1p, g_te.v1p, sizeof(double) * 3 * (size_t)nu, NULL) != 0) { rc = CCX_ERR_DEVICE; goto done; }
     {
         double e = 0.0;
         for (int64_t a = 0; a < nu; ++a) e += x3[a];
@@ -1024,8 +1236,7 @@ static int t1wt3_host(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk
         for (int r = 0; r < 3; ++r)
             for (int64_t a = 0; a < nu; ++a) v1[a + nu * rows[r]] += vfac * v1p[3 * a + r];
     }
-fail:
-    ccx_hip_free(d_w); ccx_hip_free(d_voe); ccx_hip_free(d_ep); ccx_hip_free(d_x3); ccx_hip_free(d_v1p);
+done:   /* only ep[0..nu) was overwritten: the padding the tuple kernel reads stays finite */
     free(x3); free(v1p);
     return rc;
 }
@@ -1042,28 +1253,6 @@ void t1wt3_ijk_(int64_t *i, int64_t *j, int64_t *k, int64_t *no, int64_t *nu, co
     g_status = t1wt3_host(*no, *nu, *i - 1, *j - 1, *k - 1, v3, voe6, t1, eh, ep, cf, 1.0);
 }
 
-/* one tuple of the driver loop: W into v3, then etd and the v1 rows; efac/vfac are the
- * tuple's weights (1 distinct; 1/2 for (i,j,j) and (i,i,j), whose six orderings are the
- * three distinct ones twice) */
-static int tuple_step(int64_t no, int64_t nu, int64_t ii, int64_t jj, int64_t kk, double *v1, const double *t2,
-                      const double *vm, double *v3, const double *voe, const double *eh, const double *ep,
-                      const double *ve_i, const double *ve_j, const double *ve_k, double fac)
-{
-    const int64_t nu2 = nu * nu, nou = no * nu;
-    const double *t2x[3], *vmx[6], *voex[6];
-    const int64_t xs[3] = {ii, jj, kk};
-    for (int p = 0; p < 3; ++p) t2x[p] = t2 + nu2 * no * xs[p];            /* T2(:,:,:,x) */
-    const int64_t px[6] = {ii, jj, ii, kk, jj, kk}, py[6] = {jj, ii, kk, ii, kk, jj};
-    for (int p = 0; p < 6; ++p) {
-        vmx[p] = vm + nou * (px[p] + no * py[p]);                          /* VM(:,:,x,y) */
-        voex[p] = voe + nu2 * (px[p] + no * py[p]);                        /* VOE(:,:,x,y) */
-    }
-    int rc = tuple_w_host(no, nu, ii, jj, kk, t2x[0], t2x[1], t2x[2], vmx[0], vmx[1], vmx[2], vmx[3], vmx[4], vmx[5],
-                          ve_i, ve_j, ve_k, v3);
-    if (rc == CCX_OK) rc = t1wt3_host(no, nu, ii, jj, kk, v3, voex, v1, eh, ep, fac, fac);
-    return rc;
-}
-
 /* ddcc_t_ijk_acc(no,nu,i,j,k,t1,t2,vm,v3,voe,eh,ep,ve_i,ve_j,ve_k): src/triples-ijk/ddcc_t_ijk_big.f:1-21.
  * The argument named t1 is the v1 accumulator (nu,no), as the driver passes it. */
 void ddcc_t_ijk_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, int64_t *k, double *t1, double *t2,
@@ -1074,7 +1263,10 @@ void ddcc_t_ijk_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, int64_t *
         g_status = CCX_ERR_ARG;
         return;
     }
-    g_status = tuple_step(*no, *nu, *i - 1, *j - 1, *k - 1, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_j, ve_k, 1.0);
+    const int64_t x[3] = {*i - 1, *j - 1, *k - 1};
+    if (x[0] == x[1] || x[1] == x[2] || x[0] == x[2]) { g_status = CCX_ERR_ARG; return; }
+    const double *src[3] = {ve_i, ve_j, ve_k};
+    g_status = te_full_tuple(*no, *nu, x, 1, src, 0, t2, vm, voe, eh, ep, &g_etd, t1, v3);
 }
 
 /* ddcc_t_ijj_acc(no,nu,i,j,t1,t2,vm,v3,t3,voe,o1,eh,ep,temp,ve_i,ve_j) and the (i,i,j)
@@ -1090,7 +1282,10 @@ void ddcc_t_ijj_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t
         return;
     }
     (void)t3; (void)o1; (void)temp;
-    g_status = tuple_step(*no, *nu, *i - 1, *j - 1, *j - 1, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_j, ve_j, 0.5);
+    const int64_t x[3] = {*i - 1, *j - 1, *j - 1};
+    if (x[0] == x[1]) { g_status = CCX_ERR_ARG; return; }
+    const double *src[3] = {ve_i, ve_j, NULL};
+    g_status = te_full_tuple(*no, *nu, x, 2, src, 0, t2, vm, voe, eh, ep, &g_etd, t1, v3);
 }
 
 void ddcc_t_iij_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t1, double *t2, double *vm,
@@ -1102,7 +1297,10 @@ void ddcc_t_iij_acc_(int64_t *no, int64_t *nu, int64_t *i, int64_t *j, double *t
         return;
     }
     (void)t3; (void)o1; (void)temp;
-    g_status = tuple_step(*no, *nu, *i - 1, *i - 1, *j - 1, t1, t2, vm, v3, voe, eh, ep, ve_i, ve_i, ve_j, 0.5);
+    const int64_t x[3] = {*i - 1, *i - 1, *j - 1};
+    if (x[0] == x[2]) { g_status = CCX_ERR_ARG; return; }
+    const double *src[3] = {ve_i, NULL, ve_j};
+    g_status = te_full_tuple(*no, *nu, x, 3, src, 0, t2, vm, voe, eh, ep, &g_etd, t1, v3);
 }
 
 /* ------------------------------------------------------------------ permutations (host) */
@@ -1260,15 +1458,13 @@ static struct {
     int64_t no, nu;
     double *v1;
     const double *eh, *ep, *t2, *vm, *voe;
-    double *acc_v1, *ve[3], *v3;
+    double *acc_v1;
     double etd;
 } g_gpu;
 
 static void gpu_trio_free(void)
 {
     free(g_gpu.acc_v1);
-    for (int q = 0; q < 3; ++q) free(g_gpu.ve[q]);
-    free(g_gpu.v3);
     memset(&g_gpu, 0, sizeof(g_gpu));
 }
 
@@ -1276,15 +1472,10 @@ void ijk_gpu_init_(int64_t *no, int64_t *nu, double *eh, double *ep, double *v1,
 {
     if (!no || !nu || *no < 1 || *nu < 1 || !eh || !ep || !v1 || !t2 || !vm || !voe) { g_status = CCX_ERR_ARG; return; }
     gpu_trio_free();
-    const size_t nu3 = (size_t)(*nu * *nu * *nu);
     g_gpu.acc_v1 = (double *)calloc((size_t)(*no * *nu), sizeof(double));
-    for (int q = 0; q < 3; ++q) g_gpu.ve[q] = (double *)malloc(sizeof(double) * nu3);
-    g_gpu.v3 = (double *)malloc(sizeof(double) * nu3);
-    if (!g_gpu.acc_v1 || !g_gpu.ve[0] || !g_gpu.ve[1] || !g_gpu.ve[2] || !g_gpu.v3) {
-        gpu_trio_free();
-        g_status = CCX_ERR_ALLOC;
-        return;
-    }
+    if (!g_gpu.acc_v1) { g_status = CCX_ERR_ALLOC; return; }
+    int rc = te_ensure(*no, *nu);
+    if (rc != CCX_OK) { gpu_trio_free(); g_status = rc; return; }
     g_gpu.no = *no; g_gpu.nu = *nu;
     g_gpu.v1 = v1; g_gpu.eh = eh; g_gpu.ep = ep; g_gpu.t2 = t2; g_gpu.vm = vm; g_gpu.voe = voe;
     g_gpu.etd = 0.0;
@@ -1301,20 +1492,12 @@ void ijk_gpu_driver_(int64_t *nu, int64_t *no, int64_t *i, int64_t *j, int64_t *
         g_status = CCX_ERR_ARG;
         return;
     }
-    const int64_t n = g_gpu.nu, nutr = n * (n + 1) / 2;
-    const double *patch[3] = {ve_i, ve_j, ve_k};
-    for (int s = 0; s < 3; ++s)   /* VE_x(d,p,q) = VVVO(d,q,p,x) = patch(tri(d,q), p) */
-        for (int64_t q = 0; q < n; ++q)
-            for (int64_t p = 0; p < n; ++p)
-                for (int64_t d = 0; d < n; ++d) {
-                    const int64_t hi = d > q ? d : q, lo = d > q ? q : d;
-                    g_gpu.ve[s][d + n * (p + n * q)] = patch[s][hi * (hi + 1) / 2 + lo + nutr * p];
-                }
-    const double etd0 = g_etd;
-    g_status = tuple_step(g_gpu.no, n, *i - 1, *j - 1, *k - 1, g_gpu.acc_v1, g_gpu.t2, g_gpu.vm, g_gpu.v3,
-                          g_gpu.voe, g_gpu.eh, g_gpu.ep, g_gpu.ve[0], g_gpu.ve[1], g_gpu.ve[2], 1.0);
-    g_gpu.etd += g_etd - etd0;
-    g_etd = etd0;
+    const int64_t x[3] = {*i - 1, *j - 1, *k - 1};
+    if (x[0] == x[1] || x[1] == x[2] || x[0] == x[2]) { g_status = CCX_ERR_ARG; return; }
+    /* the raw packed patches: the device expands them and swaps the index pair */
+    const double *src[3] = {ve_i, ve_j, ve_k};
+    g_status = te_full_tuple(g_gpu.no, g_gpu.nu, x, 1, src, 1, g_gpu.t2, g_gpu.vm, g_gpu.voe, g_gpu.eh, g_gpu.ep,
+                             &g_gpu.etd, g_gpu.acc_v1, NULL);
 }
 
 void ijk_gpu_finalize_(int64_t *no, int64_t *nu, double *etd)

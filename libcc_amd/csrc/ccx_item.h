@@ -141,10 +141,11 @@ struct TriplesArgs {
     int no, nu, nup, nb, kp, ntup, nblk;
     int block_major;       // work order: 1 block triple major, 0 tuple major
     long long nitems;
-    // split BF16 form: hi/mid/lo bf16 slices of AX (both orientations) and BX in
-    // v_mfma_f32_16x16x32_bf16 fragment order, K padded to kps (a multiple of 64);
-    // W = (FP32 accumulation of the slice products) * wscale
+    // split forms: fixed-point digit slices of AX (both orientations) and BX in the fragment
+    // order of their MFMA (run_phase_split), K padded to kps (two K blocks of the form);
+    // W = (g0 + g1) * wscale with wscale = sigma_A sigma_B
     const void *axr_s, *axc_s, *bx_s;
+    const int *axr_e, *axc_e, *bx_e;   // FP8 form: per-lane block exponent words
     int kps;
     double wscale;
 };
@@ -448,7 +449,7 @@ __device__ __forceinline__ void item_w(const TriplesArgs &g, double *lds, const 
     complete_orderings<MODE, T>(lds, acc, wave, lane);
 }
 
-// ---- split BF16 K sweep -----------------------------------------------------------
+// ---- tables shared by the LDS-staged sweeps --------------------------------------------
 // n-th set bit of a mask (compile time)
 __host__ __device__ constexpr int nth_bit(int mask, int n)
 {
@@ -478,21 +479,92 @@ __host__ __device__ constexpr SlotTerms slot_terms(int phase, int mode, int s)
     return r;
 }
 
-typedef const __attribute__((address_space(1))) bf16x8 gbf8;   // global memory, 16 B per lane
+// ---- split K sweeps ------------------------------------------------------------------
+// Each FP64 operand element is carried as NS low-precision slices whose sum reproduces it; the
+// slice products of levels i + j <= NS - 1 run on the BF16 or FP8 matrix cores with FP32
+// accumulation, and W is formed in FP64.  Three slicings (SplitCfg):
+//   FK 2, BF16 x 3, per-element: slice s is the bf16 rounding of what slices 0..s-1 left, so
+//     every element keeps 24 bits relative to its own magnitude; one FP32 accumulator.
+//   FK 3, FP8 e4m3 x 4, per-row block scaled: each 32-K group of a row (one lane's fragment) has
+//     its own power-of-two exponent e; x / 2^e = sum_s d_s 2^-4(s+1) with integer digits
+//     |d_0| <= 16, |d_s| <= 8 stored as e4m3, and the slice weight 2^(e - 4(s+1)) rides on the
+//     E8M0 block scales of v_mfma_scale_f32_16x16x128_f8f6f4 (a per-lane word holds the four
+//     biased slice exponents, selected by the MFMA's byte select); one FP32 accumulator.
+//   FK 4, BF16 x 4, fixed point under one power-of-two scale per slab: slice s = d_s 2^-8(s+1)
+//     (|d_0| <= 256, |d_s| <= 128); the level-0 products all lie on the grid 2^-16 sigma_A
+//     sigma_B and go to their own FP32 accumulator, which adds them exactly while its value
+//     stays below 2^8 sigma_A sigma_B; the higher levels (each at most 2^-7 of level 0) go to a
+//     second one.  W = sigma_A sigma_B (g0 + g1).
+template <int FK> struct SplitCfg {
+    static_assert(FK >= 2 && FK <= 4, "split form");
+    static constexpr int FMT = FK == 3 ? 1 : 0;       // 0 bf16, 1 fp8 e4m3
+    static constexpr int NS = FK == 2 ? 3 : 4;        // slices
+    static constexpr bool TWO_ACC = FK == 4;          // level 0 in its own accumulator
+    static constexpr bool MX = FK == 3;               // per-lane block exponents
+    static constexpr int KB = FMT == 0 ? 32 : 128;    // K per MFMA
+    static constexpr int F16 = FMT == 0 ? 1 : 2;      // 16-byte units per lane per fragment
+};
 
-// Blocks of 32 K: the BX slices of the phase (shared by all waves) are staged in LDS, double
-// buffered across blocks; the AX slices stream per operand slot through a two-deep register
-// ring (two blocks per loop iteration keep the ring index a compile-time constant).  Each
-// slot step reads the BX fragments of its terms from LDS first, then issues the 6 slice
-// products i + j <= 2 of every term and batch, smallest first, with consecutive MFMAs on
-// different accumulators.  Loads are global-address-space with one uniform base per operand,
-// so they count only against the vector-memory counter and never delay an LDS wait.
-template <int PHASE, int MODE>
-__device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const ItemCtx &it, f4 (&acc)[6][BPW],
-                                                __bf16 *lbx, int wave, int lane)
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef const __attribute__((address_space(1))) i32x4 g16;   // global memory, 16 B per lane
+typedef const __attribute__((address_space(1))) int g4;      // global memory, 4 B per lane
+
+// One lane's operand fragment: 16 B (bf16 x 8) or 32 B (e4m3 x 32), loaded as 16-byte units.
+template <int FK> using split_frag_t = typename std::conditional<SplitCfg<FK>::FMT == 0, i32x4, i32x8>::type;
+
+template <int FK>
+__device__ __forceinline__ split_frag_t<FK> split_frag_load(g16 *p, long long unit)
 {
+    if constexpr (SplitCfg<FK>::FMT == 0) {
+        return p[unit];
+    } else {
+        const i32x4 lo = p[unit], hi = p[unit + 64];   // the lane's two 16-byte units
+        return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+}
+
+template <int FK>
+__device__ __forceinline__ split_frag_t<FK> split_frag_lds(const i32x4 *p, int unit)
+{
+    if constexpr (SplitCfg<FK>::FMT == 0) {
+        return p[unit];
+    } else {
+        const i32x4 lo = p[unit], hi = p[unit + 64];
+        return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+}
+
+template <int FK, int SI, int SJ>
+__device__ __forceinline__ f4 split_mac(const split_frag_t<FK> &a, const split_frag_t<FK> &b, int ea, int eb, f4 c)
+{
+    if constexpr (SplitCfg<FK>::FMT == 0) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    } else {
+        // operand format 0 (e4m3) for A and B; byte SI of ea and byte SJ of eb are the E8M0 scales
+        return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, SI, ea, SJ, eb);
+    }
+}
+
+// Blocks of KB K: the BX slices of the phase (shared by all waves) are staged in LDS, double
+// buffered across blocks; the AX slices stream per operand slot through a two-deep register
+// ring (two blocks per loop iteration keep the ring index a compile-time constant).  A slot
+// step walks the BX slices from the last to the first, reads each from LDS just before its
+// products (so one BX slice is live at a time) and issues the products with every AX slice
+// whose level fits, for every term and batch; consecutive MFMAs go to different accumulators.
+template <int PHASE, int MODE, int FK>
+__device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const ItemCtx &it, f4 (&g0)[6][BPW],
+                                                f4 (&g1)[SplitCfg<FK>::TWO_ACC ? 6 : 1][BPW], i32x4 *lbx,
+                                                int wave, int lane)
+{
+    typedef SplitCfg<FK> CF;
     constexpr int SM = slot_mask(PHASE, MODE), XM = bx_mask(PHASE, MODE);
     constexpr int NSL = __builtin_popcount(SM), NBX = __builtin_popcount(XM);
+    constexpr int NS = CF::NS, F16 = CF::F16;
+    constexpr int CH = NS * F16;               // 16-byte chunks per lane per fragment set
+    constexpr int NCH = NBX * CH;              // BX chunks per K block
+    static_assert(2 * NCH * 1024 + 2 * NBX * 256 <= (int)(sizeof(double) * LDS_DOUBLES), "BX staging fits LDS");
     const int A = it.blk[0], B = it.blk[1], C = it.blk[2];
     int r0[2], r1[2];
     bool col[2];
@@ -500,115 +572,175 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
     else if (PHASE == 1) { r0[0] = A; r1[0] = B; col[0] = true;  r0[1] = B; r1[1] = A; col[1] = false; }
     else { r0[0] = B; r1[0] = C; col[0] = false; r0[1] = C; r1[1] = B; col[1] = true; }
     const int G = PHASE == 0 ? B : PHASE == 1 ? C : A;
-    const int nkb = g.kps / 32;
-    // units of 16 B: an AX panel is 16 outer rows x nkb blocks x 3 slices x 64 lanes = 96 kps,
-    // a BX panel nkb x 3 x 64 = 6 kps; this wave's outer rows are 2 wave and 2 wave + 1
-    gbf8 *ap[6];
+    const int nkb = g.kps / CF::KB;
+    // units of 16 B: an AX panel is 16 outer rows x nkb blocks x CH chunks x 64 lanes, a BX panel
+    // nkb x CH x 64; this wave's outer rows are 2 wave and 2 wave + 1.  Block exponents (MX): one
+    // int per lane and K block, [panel][outer][kb][lane] and [pair][G][kb][lane].
+    g16 *ap[6];
+    g4 *aep[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int o = s / 3, xp = s % 3;
         const long long panel = ((long long)it.tpos[xp] * g.nb + r0[o]) * g.nb + r1[o];
-        ap[s] = (gbf8 *)(col[o] ? g.axc_s : g.axr_s) + panel * (96LL * g.kps) +
-                (long long)wave * (2 * nkb * 192);
+        ap[s] = (g16 *)(col[o] ? g.axc_s : g.axr_s) + (panel * 16 + 2 * wave) * ((long long)nkb * CH * 64);
+        if constexpr (CF::MX) aep[s] = (g4 *)(col[o] ? g.axc_e : g.axr_e) + (panel * 16 + 2 * wave) * ((long long)nkb * 64);
     }
-    gbf8 *xq[6];
+    g16 *xq[6];
+    g4 *xe[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-        xq[q] = (gbf8 *)g.bx_s + ((long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G) * (6LL * g.kps);
-    const int lo = lane * 8;
+    for (int q = 0; q < 6; ++q) {
+        const long long pg = (long long)(it.ys[q] * g.no + it.zs[q]) * g.nb + G;
+        xq[q] = (g16 *)g.bx_s + pg * ((long long)nkb * CH * 64);
+        if constexpr (CF::MX) xe[q] = (g4 *)g.bx_e + pg * ((long long)nkb * 64);
+    }
+    int *lbe = reinterpret_cast<int *>(lbx + 2 * NCH * 64);   // BX block exponents, [buf][rank][lane]
 
-    // BX staging: chunk c = rank * 3 + slice (64 lanes x 16 B) is loaded by wave c % NWAVE
-    bf16x8 br[3];
-    auto bx_load = [&](int kb) __attribute__((always_inline)) {
-        static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
-            constexpr int c = decltype(Cc)::value;
-            constexpr int q = nth_bit(XM, c / 3), sl = c % 3;
-            if (wave == c % NWAVE) br[c / NWAVE] = xq[q][(kb * 3 + sl) * 64 + lane];
+    // BX staging: chunk c = rank * CH + (slice * F16 + f) of K block kb goes straight from global
+    // memory into LDS buffer buf (global_load_lds_dwordx4: lane l's 16 bytes land at the chunk's
+    // base + 16 l), loaded by wave c % NWAVE (wave w takes chunks w, w + NWAVE, ...); the
+    // exponent word of rank q by wave q.  No registers hold the staged data.  The rank of a chunk
+    // is wave-uniform, so its panel pointer is a uniform select.  Completion is waited for
+    // (vmcnt 0) before the barrier that publishes the buffer.
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef const __attribute__((address_space(1))) void g_void;
+    constexpr int NBR = (NCH + NWAVE - 1) / NWAVE;
+    auto bx_src = [&](int rank) __attribute__((always_inline)) {
+        g16 *p = xq[nth_bit(XM, 0)];
+        static_for<NBX>([&](auto R) __attribute__((always_inline)) {
+            constexpr int r = decltype(R)::value;
+            if (rank == r) p = xq[nth_bit(XM, r)];
         });
+        return p;
     };
-    auto bx_store = [&](int buf) __attribute__((always_inline)) {
-        static_for<NBX * 3>([&](auto Cc) __attribute__((always_inline)) {
-            constexpr int c = decltype(Cc)::value;
-            if (wave == c % NWAVE) *reinterpret_cast<bf16x8 *>(lbx + ((buf * NBX) * 3 + c) * 512 + lo) = br[c / NWAVE];
-        });
-    };
-    bf16x8 ab[2][BPW][3];
-    auto a_load = [&](int s, int kb, bf16x8 (&d)[BPW][3]) __attribute__((always_inline)) {
+    auto bx_load = [&](int kb, int buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int bb = 0; bb < BPW; ++bb)
-#pragma unroll
-            for (int sl = 0; sl < 3; ++sl) d[bb][sl] = ap[s][((bb * nkb + kb) * 3 + sl) * 64 + lane];
+        for (int r = 0; r < NBR; ++r) {
+            const int c = wave + NWAVE * r;
+            if (c < NCH)
+                __builtin_amdgcn_global_load_lds((g_void *)(bx_src(c / CH) + ((long long)kb * CH + c % CH) * 64 + lane),
+                                                 (lds_void *)(lbx + (buf * NCH + c) * 64), 16, 0, 0);
+        }
+        if constexpr (CF::MX) {
+            if (wave < NBX) {
+                g4 *e = xe[nth_bit(XM, 0)];
+                static_for<NBX>([&](auto R) __attribute__((always_inline)) {
+                    constexpr int r = decltype(R)::value;
+                    if (wave == r) e = xe[nth_bit(XM, r)];
+                });
+                __builtin_amdgcn_global_load_lds((g_void *)(e + (long long)kb * 64 + lane),
+                                                 (lds_void *)(lbe + (buf * NBX + wave) * 64), 4, 0, 0);
+            }
+        }
     };
-    auto compute = [&](auto Sc, const bf16x8 (&a)[BPW][3], int buf) __attribute__((always_inline)) {
-        constexpr int s = decltype(Sc)::value;
+    // A ring step is one operand slot and BS of the wave's batches: both for bf16, one at a time
+    // for the 32-byte fp8 fragments (the registers of a two-batch step would not fit next to the
+    // accumulators).
+    typedef split_frag_t<FK> frag_t;
+    constexpr int BS = CF::FMT == 0 ? BPW : 1, NH = BPW / BS;
+    frag_t ab[2][BS][NS];
+    int ae[2][BS];
+    auto a_load = [&](int s, int h, int kb, frag_t (&d)[BS][NS], int (&de)[BS]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int b = 0; b < BS; ++b) {
+            const int bb = h * BS + b;
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl)
+                d[b][sl] = split_frag_load<FK>(ap[s], (((long long)bb * nkb + kb) * CH + sl * F16) * 64 + lane);
+            if constexpr (CF::MX) de[b] = aep[s][((long long)bb * nkb + kb) * 64 + lane];
+        }
+    };
+    auto compute = [&](auto Sc, auto Hc, const frag_t (&a)[BS][NS], const int (&ea)[BS], int buf)
+                       __attribute__((always_inline)) {
+        constexpr int s = decltype(Sc)::value, h = decltype(Hc)::value;
         constexpr SlotTerms ST = slot_terms(PHASE, MODE, s);
-        bf16x8 bs[ST.n > 0 ? ST.n : 1][3];
+        int eb[ST.n > 0 ? ST.n : 1];
         static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
             constexpr int n = decltype(N)::value;
             constexpr int rank = __builtin_popcount(XM & ((1 << ST.b[n]) - 1));
-#pragma unroll
-            for (int sl = 0; sl < 3; ++sl)
-                bs[n][sl] = *reinterpret_cast<const bf16x8 *>(lbx + ((buf * NBX + rank) * 3 + sl) * 512 + lo);
+            eb[n] = CF::MX ? lbe[(buf * NBX + rank) * 64 + lane] : 0;
         });
-        // slice products with i + j <= 2, smallest first
-        static_for<6>([&](auto Q) __attribute__((always_inline)) {
-            constexpr int qq = 5 - decltype(Q)::value;
-            constexpr int si = qq == 0 ? 0 : qq == 1 ? 0 : qq == 2 ? 1 : qq == 3 ? 0 : qq == 4 ? 1 : 2;
-            constexpr int sj = qq == 0 ? 0 : qq == 1 ? 1 : qq == 2 ? 0 : qq == 3 ? 2 : qq == 4 ? 1 : 0;
+        static_for<NS>([&](auto SJ) __attribute__((always_inline)) {
+            constexpr int sj = NS - 1 - decltype(SJ)::value;     // BX slice, last first
+            // A memory clobber per BX slice: without it the compiler merges the LDS reads of the
+            // same slice across the steps of a block and keeps them all live (for the fp8 slices
+            // that alone exceeds the register file); each step re-reads its slice instead.
+            asm volatile("" ::: "memory");
+            frag_t bs[ST.n > 0 ? ST.n : 1];
             static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
                 constexpr int n = decltype(N)::value;
-                constexpr int p = ST.p[n];
+                constexpr int rank = __builtin_popcount(XM & ((1 << ST.b[n]) - 1));
+                bs[n] = split_frag_lds<FK>(lbx, (buf * NCH + rank * CH + sj * F16) * 64 + lane);
+            });
+            static_for<NS - sj>([&](auto SI) __attribute__((always_inline)) {
+                constexpr int si = NS - 1 - sj - decltype(SI)::value;   // AX slice, highest level first
+                static_for<ST.n>([&](auto N) __attribute__((always_inline)) {
+                    constexpr int n = decltype(N)::value;
+                    constexpr int p = ST.p[n];
 #pragma unroll
-                for (int bb = 0; bb < BPW; ++bb) {
-                    if constexpr (PHASE == 2)
-                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bs[n][sj], a[bb][si], acc[p][bb], 0, 0, 0);
-                    else
-                        acc[p][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[bb][si], bs[n][sj], acc[p][bb], 0, 0, 0);
-                }
+                    for (int b = 0; b < BS; ++b) {
+                        const int bb = h * BS + b;
+                        f4 &acc = (CF::TWO_ACC && si + sj > 0) ? g1[CF::TWO_ACC ? p : 0][bb] : g0[p][bb];
+                        // PHASE 2 tiles are produced transposed (BX rows, AX columns)
+                        if constexpr (PHASE == 2) acc = split_mac<FK, sj, si>(bs[n], a[b][si], eb[n], ea[b], acc);
+                        else acc = split_mac<FK, si, sj>(a[b][si], bs[n], ea[b], eb[n], acc);
+                    }
+                });
             });
         });
     };
 
-    bx_load(0);
-    bx_store(0);
+    // steps of one K block: (slot j, batch group h), j major; NST = NSL * NH per block
+    constexpr int NST = NSL * NH;
+    bx_load(0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    a_load(nth_bit(SM, 0), 0, ab[0]);
+    a_load(nth_bit(SM, 0), 0, 0, ab[0], ae[0]);
     for (int kb = 0; kb < nkb; kb += 2) {
         static_for<2>([&](auto HB) __attribute__((always_inline)) {
             constexpr int hb = decltype(HB)::value;
             const int kbb = kb + hb;
-            bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1);
-            static_for<NSL>([&](auto J) __attribute__((always_inline)) {
+            // buffer 1 - hb was last read in the previous block, which ended with a barrier
+            bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1, 1 - hb);
+            static_for<NST>([&](auto J) __attribute__((always_inline)) {
                 constexpr int j = decltype(J)::value;
-                constexpr int t = hb * NSL + j;
-                constexpr int s = nth_bit(SM, j);
-                constexpr int sn = nth_bit(SM, (j + 1) % NSL);
-                const int kn = j + 1 < NSL ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
-                a_load(sn, kn, ab[(t + 1) % 2]);
+                constexpr int t = hb * NST + j;
+                constexpr int s = nth_bit(SM, j / NH), h = j % NH;
+                constexpr int sn = nth_bit(SM, ((j + 1) % NST) / NH), hn = ((j + 1) % NST) % NH;
+                const int kn = j + 1 < NST ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+                a_load(sn, hn, kn, ab[(t + 1) % 2], ae[(t + 1) % 2]);
                 __builtin_amdgcn_sched_barrier(0);
-                compute(std::integral_constant<int, s>{}, ab[t % 2], hb);
+                compute(std::integral_constant<int, s>{}, std::integral_constant<int, h>{}, ab[t % 2], ae[t % 2], hb);
                 __builtin_amdgcn_sched_barrier(0);
             });
-            bx_store(1 - hb);
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         });
     }
 }
 
-template <int MODE>
+// W of an item in a split form: g0 (and g1 for the two-accumulator form) on return hold
+// w_p = (g0 + g1) * wscale at (row(lane,v), wave*BPW+bb, lane&15) in the FP32 MFMA layout.
+template <int MODE, int FK>
 __device__ __forceinline__ void item_w_split(const TriplesArgs &g, double *lds, const ItemCtx &it,
-                                             f4 (&acc)[6][BPW], int wave, int lane)
+                                             f4 (&g0)[6][BPW], f4 (&g1)[SplitCfg<FK>::TWO_ACC ? 6 : 1][BPW],
+                                             int wave, int lane)
 {
+    constexpr bool TWO = SplitCfg<FK>::TWO_ACC;
 #pragma unroll
     for (int p = 0; p < 6; ++p)
 #pragma unroll
-        for (int bb = 0; bb < BPW; ++bb) acc[p][bb] = f4{0, 0, 0, 0};
-    __bf16 *lbx = reinterpret_cast<__bf16 *>(lds);
-    run_phase_split<0, MODE>(g, it, acc, lbx, wave, lane);
-    transpose_lb_to_l<MODE, float>(lds, acc, wave, lane);
-    run_phase_split<1, MODE>(g, it, acc, lbx, wave, lane);
-    run_phase_split<2, MODE>(g, it, acc, lbx, wave, lane);
-    complete_orderings<MODE, float>(lds, acc, wave, lane);
+        for (int bb = 0; bb < BPW; ++bb) {
+            g0[p][bb] = f4{0, 0, 0, 0};
+            if constexpr (TWO) g1[p][bb] = f4{0, 0, 0, 0};
+        }
+    i32x4 *lbx = reinterpret_cast<i32x4 *>(lds);
+    run_phase_split<0, MODE, FK>(g, it, g0, g1, lbx, wave, lane);
+    transpose_lb_to_l<MODE, float>(lds, g0, wave, lane);
+    if constexpr (TWO) transpose_lb_to_l<MODE, float>(lds, g1, wave, lane);
+    run_phase_split<1, MODE, FK>(g, it, g0, g1, lbx, wave, lane);
+    run_phase_split<2, MODE, FK>(g, it, g0, g1, lbx, wave, lane);
+    complete_orderings<MODE, float>(lds, g0, wave, lane);
+    if constexpr (TWO) complete_orderings<MODE, float>(lds, g1, wave, lane);
 }
 
 // ---- FP32 K sweep -------------------------------------------------------------------

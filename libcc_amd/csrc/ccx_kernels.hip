@@ -47,13 +47,15 @@ __device__ __forceinline__ double ax_value(const double *__restrict__ vvvo, cons
     return 0.0;
 }
 
-// COLS = 0: row layout [..][alpha][kb][k4][beta]; COLS = 1: column layout [..][beta][kb][k4][alpha]
+// COLS = 0: row layout [..][alpha][kb][k4][beta]; COLS = 1: column layout [..][beta][kb][k4][alpha].
+// Packs the panels of nx occupied indices x = 0..nx-1 from vvvo + x nu^3 and t2 + x nu^2 no (nx = no
+// for the whole slab; 1 for one index, with the pointers at that index).
 template <int COLS, class T>
 __global__ void pack_ax_kernel(const double *__restrict__ vvvo, const double *__restrict__ t2,
-                               T *__restrict__ ax, int no, int nu, int nb, int kp)
+                               T *__restrict__ ax, int no, int nu, int nb, int kp, int nx)
 {
     const long long per_panel = 256LL * kp;
-    const long long total = (long long)no * nb * nb * per_panel;
+    const long long total = (long long)nx * nb * nb * per_panel;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
         long long r = idx % per_panel;
@@ -99,6 +101,37 @@ __global__ void pack_bx_kernel(const double *__restrict__ t2, const double *__re
     }
 }
 
+// BX of one ordered pair (y,z): BX[y,z](g,K) = T2(g,K,y,z) for K < nu, -VM(K-nu,g,z,y) for K < nu+no;
+// t2yz = T2(:,:,y,z) (nu^2), vmzy = VM(:,:,z,y) (no*nu), bx = the pair's [nb][kp][16] panel set.
+__global__ void pack_bx_pair_kernel(const double *__restrict__ t2yz, const double *__restrict__ vmzy,
+                                    double *__restrict__ bx, int no, int nu, int nb, int kp)
+{
+    const long long total = (long long)nb * kp * 16;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int gl = (int)(idx & 15);
+        const int K = (int)((idx >> 4) % kp);
+        const int g = (int)((idx >> 4) / kp) * TB + gl;
+        double v = 0.0;
+        if (g < nu) {
+            if (K < nu) v = t2yz[g + (long long)nu * K];
+            else if (K < nu + no) v = -vmzy[(K - nu) + (long long)no * g];
+        }
+        bx[idx] = v;
+    }
+}
+
+// VVVO(d,q,p,x) = VE_x(d,p,q): the (2,3) index swap of one occupied index's cube
+__global__ void ve_to_vvvo_kernel(const double *__restrict__ ve, double *__restrict__ vvvo, int nu)
+{
+    const long long n3 = (long long)nu * nu * nu;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n3;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long d = idx % nu, q = (idx / nu) % nu, p = idx / ((long long)nu * nu);
+        vvvo[idx] = ve[d + nu * (p + (long long)nu * q)];
+    }
+}
+
 __global__ void pack_voe_kernel(const double *__restrict__ voe, double *__restrict__ out, int no,
                                 int nu, int nup)
 {
@@ -135,7 +168,7 @@ __global__ void unpack_vvvo_kernel(const double *__restrict__ packed, double *__
 }
 
 // ---------------------------------------------------------------------------------
-// split BF16 form: fixed-point slices of the FP64 operand slabs
+// split forms: fixed-point digit slices of the FP64 operand slabs
 // ---------------------------------------------------------------------------------
 
 // max |x| over a slab (non-negative doubles order like their bit patterns)
@@ -148,56 +181,96 @@ __global__ void absmax_kernel(const double *__restrict__ x, long long n, unsigne
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-// x -> x0 + x1 + x2, each slice the bf16 rounding of what the previous ones left: every
-// slice is exact in bf16 (8 significant bits) and the three carry about 26 bits of x
-// relative to |x| itself, so small elements keep their precision next to large ones.  A
-// product of two slices has at most 16 significant bits and is exact in FP32; the dropped
-// products (x1 y2, x2 y1, x2 y2) and the remainder are below 2^-26 |x y|.
-__device__ __forceinline__ void split3(double x, __bf16 &a0, __bf16 &a1, __bf16 &a2)
+// OCP e4m3 byte of an integer |d| <= 16 (every such integer has at most 4 significant bits)
+__device__ __forceinline__ unsigned char e4m3_of_int(int d)
 {
-    a0 = (__bf16)(float)x;
-    double r = x - (double)(float)a0;
-    a1 = (__bf16)(float)r;
-    r -= (double)(float)a1;
-    a2 = (__bf16)(float)r;
+    if (d == 0) return 0;
+    const unsigned char sgn = d < 0 ? 0x80 : 0;
+    const int m = d < 0 ? -d : d;
+    const int p = 31 - __builtin_clz(m);          // 2^p <= m < 2^(p+1)
+    const int frac = ((m << 3) >> p) & 7;         // the three bits after the leading one
+    return sgn | (unsigned char)((p + 7) << 3) | (unsigned char)frac;
 }
 
-// Split slabs in v_mfma_f32_16x16x32_bf16 fragment order, the three slices of one K block
-// side by side: AX [panel][outer][K block of 32][slice][lane][e], BX [panel][K block][slice]
-// [lane][e], with inner index = lane & 15 and K = 32 kb + 8 (lane >> 4) + e.  One panel is one
-// contiguous range, so a wave reaches all its fragments from one uniform base pointer.  BX = 1:
-// the source is the BX slab [panel][K][16 gamma] and there is no outer index.
-template <int BX>
-__global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__restrict__ dst, long long npanel,
-                                  int kp, int kps, double inv_scale)
+// Slices of a slab in the fragment order of the form's MFMA (see run_phase_split).  One thread
+// per fragment group: the EL elements one lane holds for one K block of one row (AX: row = the
+// panel's outer index; BX: no outer index), K = KB kb + EL (lane >> 4) + e, inner index lane & 15.
+// The MFMA sums over every K of a block, so any placement of K inside a block that A and B share
+// gives the same products.  Output: [panel][outer][kb][slice][16-byte unit f][lane][16 B]; the FP8
+// form also writes one exponent word per group, [panel][outer][kb][lane].  Source: the FP64 slab,
+// AX [panel][outer/2][K/4][lane 64][2] or BX [panel][K][16 gamma].  inv_scale = 1 / sigma of the slab.
+template <int FK, int BX>
+__global__ void pack_split_kernel(const double *__restrict__ src, unsigned char *__restrict__ dst,
+                                  int *__restrict__ dexp, long long npanel, int kp, int kps, double inv_scale)
 {
-    const int nkb = kps / 32;
-    const long long per_panel = (BX ? 16LL : 256LL) * kps;
-    const long long total = npanel * per_panel;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(idx & 7), lane = (int)((idx >> 3) & 63);
-        long long rr = idx >> 9;           // (panel, outer, kb) or (panel, kb)
+    typedef SplitCfg<FK> CF;
+    constexpr int NS = CF::NS, F16 = CF::F16, KB = CF::KB;
+    constexpr int EL = CF::FMT == 0 ? 8 : 32, EB = CF::FMT == 0 ? 2 : 1;
+    const int nkb = kps / KB;
+    const long long ngroups = npanel * (BX ? 1 : 16) * nkb * 64;
+    for (long long gi = blockIdx.x * (long long)blockDim.x + threadIdx.x; gi < ngroups;
+         gi += (long long)gridDim.x * blockDim.x) {
+        const int lane = (int)(gi & 63);
+        const long long rr = gi >> 6;      // (panel, outer, kb) or (panel, kb)
         const int kb = (int)(rr % nkb);
         const long long po = rr / nkb;     // panel * 16 + outer, or panel
-        const int K = kb * 32 + (lane >> 4) * 8 + e, inner = lane & 15;
-        double v = 0.0;
-        if (K < kp) {
-            if (BX) {
-                v = src[po * (16LL * kp) + (long long)K * 16 + inner];
-            } else {
-                const int outer = (int)(po % 16);
-                const long long panel = po / 16;
-                v = src[panel * (256LL * kp) + (((long long)(outer >> 1) * (kp / 4) + K / 4) * 64 + (K & 3) * 16 + inner) * 2 +
-                        (outer & 1)];
+        const int inner = lane & 15;
+        double v[EL];
+        double m = 0.0;
+#pragma unroll
+        for (int e = 0; e < EL; ++e) {
+            const int K = kb * KB + (lane >> 4) * EL + e;
+            double x = 0.0;
+            if (K < kp) {
+                if (BX) {
+                    x = src[po * (16LL * kp) + (long long)K * 16 + inner];
+                } else {
+                    const int outer = (int)(po % 16);
+                    const long long panel = po / 16;
+                    x = src[panel * (256LL * kp) + (((long long)(outer >> 1) * (kp / 4) + K / 4) * 64 + (K & 3) * 16 + inner) * 2 +
+                            (outer & 1)];
+                }
+            }
+            v[e] = x * inv_scale;          // |v| <= 1
+            m = fmax(m, fabs(v[e]));
+        }
+        int gexp = 0;                      // FP8 form: group exponent, |v| <= 2^gexp
+        if constexpr (CF::MX) {
+            if (m > 0.0) frexp(m, &gexp);
+            if (gexp < -100) gexp = -100;  // a group this small is carried as zero digits
+        }
+#pragma unroll
+        for (int e = 0; e < EL; ++e) {
+            const int f = (e * EB) / 16, byte = (e * EB) % 16;
+            double t = v[e];
+            if constexpr (CF::MX) t = ldexp(t, -gexp);
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                const long long o = (((rr * NS + sl) * F16 + f) * 64 + lane) * 16 + byte;
+                if constexpr (FK == 2) {          // per element: bf16 rounding of what is left
+                    const __bf16 h = (__bf16)(float)t;
+                    t -= (double)(float)h;
+                    *reinterpret_cast<__bf16 *>(dst + o) = h;
+                } else if constexpr (FK == 4) {   // fixed point: digits base 2^8
+                    t *= 256.0;
+                    const double d = rint(t);
+                    t -= d;
+                    *reinterpret_cast<__bf16 *>(dst + o) = (__bf16)(float)ldexp(d, -8 * (sl + 1));   // exact
+                } else {                          // digits base 2^4 as e4m3 integers
+                    t *= 16.0;
+                    const double d = rint(t);
+                    t -= d;
+                    dst[o] = gexp > -100 ? e4m3_of_int((int)d) : 0;
+                }
             }
         }
-        __bf16 a0, a1, a2;
-        split3(v * inv_scale, a0, a1, a2);
-        const long long o = (rr * 3) * 512 + lane * 8 + e;
-        dst[o] = a0;
-        dst[o + 512] = a1;
-        dst[o + 1024] = a2;
+        if constexpr (CF::MX) {
+            // byte s: biased E8M0 exponent of slice s, 2^(gexp - 4(s+1))
+            unsigned int w = 0;
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) w |= (unsigned int)(127 + gexp - 4 * (sl + 1)) << (8 * sl);
+            dexp[rr * 64 + lane] = (int)w;
+        }
     }
 }
 
@@ -206,7 +279,7 @@ __global__ void pack_split_kernel(const double *__restrict__ src, __bf16 *__rest
 // ---------------------------------------------------------------------------------
 namespace {
 
-// FK: 0 FP64 slabs, 1 FP32 slabs (16 K per fragment load), 2 split BF16 slices.  Always
+// FK: 0 FP64 slabs, 1 FP32 slabs (16 K per fragment load), 2-4 split slices (SplitCfg).  Always
 // inlined: as a called function every item paid a call frame and callee-saved register
 // spills in scratch memory (0.06 TB of writes per launch at no=24, nu=216).
 template <int MODE, class T, int FK>
@@ -214,7 +287,8 @@ __device__ __forceinline__ void do_item(const TriplesArgs &g, double *lds, doubl
                         int tkind, int mult, int wave, int lane, int tid)
 {
     typename Mfma<T>::acc_t acc[6][BPW];
-    if constexpr (FK == 2) item_w_split<MODE>(g, lds, it, acc, wave, lane);
+    f4 acc1[FK == 4 ? 6 : 1][BPW];   // two-accumulator split form: the levels above 0
+    if constexpr (FK >= 2) item_w_split<MODE, FK>(g, lds, it, acc, acc1, wave, lane);
     else if constexpr (FK == 1) item_w_f32v<MODE>(g, lds, it, acc, wave, lane);
     else item_w<MODE, T>(g, lds, it, acc, wave, lane);
 
@@ -260,8 +334,9 @@ __device__ __forceinline__ void do_item(const TriplesArgs &g, double *lds, doubl
             double w[6];
 #pragma unroll
             for (int p = 0; p < 6; ++p) {
-                w[p] = (double)acc[p][bb][v];
-                if constexpr (FK == 2) w[p] *= g.wscale;
+                if constexpr (FK == 4) w[p] = ((double)acc[p][bb][v] + (double)acc1[p][bb][v]) * g.wscale;
+                else if constexpr (FK >= 2) w[p] = (double)acc[p][bb][v] * g.wscale;
+                else w[p] = (double)acc[p][bb][v];
             }
             const double inv = 1.0 / (dijk - g.ep[ag] - g.ep[bg] - g.ep[cg]);
             const double ev = w[0] + w[3] + w[4], od = w[1] + w[2] + w[5];
@@ -563,6 +638,29 @@ __global__ void pack_f32v_kernel(const double *__restrict__ src, float *__restri
     }
 }
 
+// grid-stride launches: enough work groups to fill the GPU, at most 65536
+static int grid_for(long long total)
+{
+    long long g = (total + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+template <int FK>
+static void launch_pack_split(const double *axr, const double *axc, const double *bx, void *const *sl, int no, int nb,
+                              int kp, int kps, const double *sc, hipStream_t s)
+{
+    constexpr int EL = SplitCfg<FK>::FMT == 0 ? 8 : 32;
+    const long long ax_g = (long long)no * nb * nb * 256 * kps / EL, bx_g = (long long)no * no * nb * 16 * kps / EL;
+    pack_split_kernel<FK, 0><<<grid_for(ax_g), 256, 0, s>>>(axr, (unsigned char *)sl[0], (int *)sl[3],
+                                                            (long long)no * nb * nb, kp, kps, 1.0 / sc[0]);
+    pack_split_kernel<FK, 0><<<grid_for(ax_g), 256, 0, s>>>(axc, (unsigned char *)sl[1], (int *)sl[4],
+                                                            (long long)no * nb * nb, kp, kps, 1.0 / sc[0]);
+    pack_split_kernel<FK, 1><<<grid_for(bx_g), 256, 0, s>>>(bx, (unsigned char *)sl[2], (int *)sl[5],
+                                                            (long long)no * no * nb, kp, kps, 1.0 / sc[1]);
+}
+
 // ---------------------------------------------------------------------------------
 // thin C ABI used by the C host layer
 // ---------------------------------------------------------------------------------
@@ -610,13 +708,6 @@ int ccx_hip_cu_count(int *n)
 
 const char *ccx_hip_last_error(void) { return hipGetErrorString(hipGetLastError()); }
 
-static int grid_for(long long total)
-{
-    long long g = (total + 255) / 256;
-    if (g > 65536) g = 65536;
-    if (g < 1) g = 1;
-    return (int)g;
-}
 
 // form 0: FP64 slabs (double); form 1: FP32 slabs (float).  voep is written only when non-null.
 int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *vm, const double *voe, void *axr,
@@ -627,47 +718,90 @@ int ccx_hip_pack(int form, const double *vvvo, const double *t2, const double *v
     const long long nax = (long long)no * nb * nb * kp * 256;
     const int gbx = grid_for((long long)no * no * nb * kp * 16);
     if (form == 0) {
-        pack_ax_kernel<0, double><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (double *)axr, no, nu, nb, kp);
-        pack_ax_kernel<1, double><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (double *)axc, no, nu, nb, kp);
+        pack_ax_kernel<0, double><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (double *)axr, no, nu, nb, kp, no);
+        pack_ax_kernel<1, double><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (double *)axc, no, nu, nb, kp, no);
         pack_bx_kernel<double><<<gbx, 256, 0, s>>>(t2, vm, (double *)bx, no, nu, nb, kp);
     } else {
-        pack_ax_kernel<0, float><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (float *)axr, no, nu, nb, kp);
-        pack_ax_kernel<1, float><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (float *)axc, no, nu, nb, kp);
+        pack_ax_kernel<0, float><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (float *)axr, no, nu, nb, kp, no);
+        pack_ax_kernel<1, float><<<grid_for(nax), 256, 0, s>>>(vvvo, t2, (float *)axc, no, nu, nb, kp, no);
         pack_bx_kernel<float><<<gbx, 256, 0, s>>>(t2, vm, (float *)bx, no, nu, nb, kp);
     }
     if (voep) pack_voe_kernel<<<grid_for((long long)no * no * nup * nup), 256, 0, s>>>(voe, voep, no, nu, nup);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// The FP64 panels of one occupied index (both AX orientations) from vvvo_x = VVVO(:,:,:,x) (nu^3) and
+// t2_x = T2(:,:,:,x) (nu^2 no); axr_x / axc_x point at that index's panels.
+int ccx_hip_pack_ax_one(const double *vvvo_x, const double *t2_x, double *axr_x, double *axc_x, int no, int nu,
+                        int nup, int kp, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = nup / TB;
+    const long long n = (long long)nb * nb * kp * 256;
+    pack_ax_kernel<0, double><<<grid_for(n), 256, 0, s>>>(vvvo_x, t2_x, axr_x, no, nu, nb, kp, 1);
+    pack_ax_kernel<1, double><<<grid_for(n), 256, 0, s>>>(vvvo_x, t2_x, axc_x, no, nu, nb, kp, 1);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// The FP64 BX panels of one ordered pair (y,z) and, when voe_yz is non-null, its padded VOE block:
+// t2_yz = T2(:,:,y,z), vm_zy = VM(:,:,z,y), voe_yz = VOE(:,:,y,z).
+int ccx_hip_pack_pair(const double *t2_yz, const double *vm_zy, const double *voe_yz, double *bx_yz,
+                      double *voep_yz, int no, int nu, int nup, int kp, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = nup / TB;
+    pack_bx_pair_kernel<<<grid_for((long long)nb * kp * 16), 256, 0, s>>>(t2_yz, vm_zy, bx_yz, no, nu, nb, kp);
+    if (voe_yz) pack_voe_kernel<<<grid_for((long long)nup * nup), 256, 0, s>>>(voe_yz, voep_yz, 1, nu, nup);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ccx_hip_ve_to_vvvo(const double *ve, double *vvvo_x, int nu, void *stream)
+{
+    ve_to_vvvo_kernel<<<grid_for((long long)nu * nu * nu), 256, 0, (hipStream_t)stream>>>(ve, vvvo_x, nu);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 // form 0: FP64 slabs; 1: FP32 slabs in the FP32 sweep's order (K padded to kps, a multiple of
 // 48); 2: split BF16 (split[] = axr_s, axc_s, bx_s, K padded to kps, W scaled by wscale)
+// K padding of a form's slabs: FP32 48; split forms two K blocks of their MFMA (64 for bf16,
+// 256 for fp8 e4m3); FP64 4
+int ccx_hip_form_kpad(int form) { return form == 1 ? 48 : form == 3 ? 256 : form >= 2 ? 64 : 4; }
+
+// form 0: FP64 slabs; 1: FP32 slabs in the FP32 sweep's order (K padded to kps, a multiple of
+// 48); 2-4: split slices (split[] = axr_s, axc_s, bx_s and, for form 3, the exponent words
+// axr_e, axc_e, bx_e; K padded to kps, W scaled by wscale)
 int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, const double *voep,
                     const double *eh, const double *ep, const int *tuples, const int *blocks, double *e_part,
                     double *v1_part, double *wout, unsigned int *counter, int no, int nu, int nup, int kp,
                     int ntup, int nblk, int grid, const void *const *split, int kps, double wscale, void *stream)
 {
-    static bool attr_set[3] = {false, false, false};
+    static bool attr_set[5] = {false, false, false, false, false};
     const size_t lds_bytes = sizeof(double) * LDS_DOUBLES;
-    if (form < 0 || form > 2) return 3;
-    if (form == 2 && (split == nullptr || kps % 64 != 0)) return 3;
+    if (form < 0 || form > 4) return 3;
+    if (form >= 2 && (split == nullptr || kps % ccx_hip_form_kpad(form) != 0)) return 3;
     if (form == 1 && kps % 48 != 0) return 3;   // FP32 slabs: K padded to 48 (kps)
+    const void *fns[5] = {(const void *)triples_kernel<double, 0>, (const void *)triples_kernel<float, 1>,
+                          (const void *)triples_kernel<float, 2>, (const void *)triples_kernel<float, 3>,
+                          (const void *)triples_kernel<float, 4>};
     if (!attr_set[form]) {
-        const void *fn = form == 0 ? (const void *)triples_kernel<double, 0>
-                       : form == 1 ? (const void *)triples_kernel<float, 1> : (const void *)triples_kernel<float, 2>;
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
+        if (hipFuncSetAttribute(fns[form], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return 2;
         attr_set[form] = true;
     }
+    const bool sp = form >= 2;
     TriplesArgs g;
     g.axr = axr; g.axc = axc; g.bx = bx; g.voe = voep; g.eh = eh; g.ep = ep; g.tuples = tuples;
     g.blocks = blocks; g.e_part = e_part; g.v1_part = v1_part; g.wout = wout; g.counter = counter;
     g.no = no; g.nu = nu; g.nup = nup; g.nb = nup / TB; g.kp = kp; g.ntup = ntup; g.nblk = nblk;
     g.nitems = (long long)ntup * nblk;
-    g.axr_s = form == 2 ? split[0] : nullptr;
-    g.axc_s = form == 2 ? split[1] : nullptr;
-    g.bx_s = form == 2 ? split[2] : nullptr;
+    g.axr_s = sp ? split[0] : nullptr;
+    g.axc_s = sp ? split[1] : nullptr;
+    g.bx_s = sp ? split[2] : nullptr;
+    g.axr_e = form == 3 ? (const int *)split[3] : nullptr;
+    g.axc_e = form == 3 ? (const int *)split[4] : nullptr;
+    g.bx_e = form == 3 ? (const int *)split[5] : nullptr;
     g.kps = kps;
-    g.wscale = form == 2 ? wscale : 1.0;
+    g.wscale = sp ? wscale : 1.0;
     {
         const char *env = getenv("CCX_ITEM_ORDER");
         g.block_major = (env && env[0] == 't') ? 0 : 1;
@@ -677,21 +811,34 @@ int ccx_hip_triples(int form, const void *axr, const void *axc, const void *bx, 
     if (grid > g.nitems) grid = (int)g.nitems;
     hipStream_t s = (hipStream_t)stream;
     zero_counter_kernel<<<1, 1, 0, s>>>(counter);
-    if (form == 0) triples_kernel<double, 0><<<grid, NTHR, lds_bytes, s>>>(g);
-    else if (form == 1) triples_kernel<float, 1><<<grid, NTHR, lds_bytes, s>>>(g);
-    else triples_kernel<float, 2><<<grid, NTHR, lds_bytes, s>>>(g);
+    switch (form) {
+    case 0: triples_kernel<double, 0><<<grid, NTHR, lds_bytes, s>>>(g); break;
+    case 1: triples_kernel<float, 1><<<grid, NTHR, lds_bytes, s>>>(g); break;
+    case 2: triples_kernel<float, 2><<<grid, NTHR, lds_bytes, s>>>(g); break;
+    case 3: triples_kernel<float, 3><<<grid, NTHR, lds_bytes, s>>>(g); break;
+    default: triples_kernel<float, 4><<<grid, NTHR, lds_bytes, s>>>(g); break;
+    }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// Split BF16 slabs from the FP64 ones: power-of-two scales sA >= max|AX|, sB >= max|BX| (they
-// keep FP64 inputs of any range inside FP32's), three bf16 slices of AX/sA (both orientations)
-// and BX/sB; *wscale = sA * sB.
-int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, void *axr_s, void *axc_s, void *bx_s,
+// Bytes per slab element of a split form's slices (BF16 x 3: 6, FP8 x 4: 4, BF16 x 4: 8); the FP8
+// form adds one exponent word per 32 elements.
+int ccx_hip_split_elem_bytes(int form) { return form == 2 ? 6 : form == 3 ? 4 : form == 4 ? 8 : 0; }
+
+// Split slabs from the FP64 ones: power-of-two scales sigma_A >= max|AX|, sigma_B >= max|BX| (the
+// fixed-point grid of the BF16 x 4 slabs; for every form they keep FP64 inputs of any range inside
+// FP32's), then the slices of AX / sigma_A (both orientations) and BX / sigma_B; sl[] = axr_s, axc_s,
+// bx_s, axr_e, axc_e, bx_e; *wscale = sigma_A sigma_B.
+int ccx_hip_pack_split(int form, const double *axr, const double *axc, const double *bx, void *const *sl,
                        unsigned long long *dmax, int no, int nb, int kp, int kps, double *wscale, void *stream)
 {
+    if (form < 2 || form > 4 || kps % ccx_hip_form_kpad(form) != 0) return 3;
     hipStream_t s = (hipStream_t)stream;
     const long long n_ax = (long long)no * nb * nb * 256 * kp, n_bx = (long long)no * no * nb * 16 * kp;
-    if (hipMemsetAsync(dmax, 0, 2 * sizeof(unsigned long long), s) != hipSuccess) return 1;
+    zero_counter_kernel<<<1, 1, 0, s>>>(reinterpret_cast<unsigned int *>(dmax));
+    zero_counter_kernel<<<1, 1, 0, s>>>(reinterpret_cast<unsigned int *>(dmax) + 1);
+    zero_counter_kernel<<<1, 1, 0, s>>>(reinterpret_cast<unsigned int *>(dmax) + 2);
+    zero_counter_kernel<<<1, 1, 0, s>>>(reinterpret_cast<unsigned int *>(dmax) + 3);
     absmax_kernel<<<grid_for(n_ax), 256, 0, s>>>(axr, n_ax, dmax);
     absmax_kernel<<<grid_for(n_bx), 256, 0, s>>>(bx, n_bx, dmax + 1);
     unsigned long long h[2];
@@ -705,10 +852,9 @@ int ccx_hip_pack_split(const double *axr, const double *axc, const double *bx, v
         if (m > 0.0) frexp(m, &e);   // m < 2^e
         sc[q] = m > 0.0 ? ldexp(1.0, e) : 1.0;
     }
-    const long long ax_n = (long long)no * nb * nb * 256 * kps, bx_n = (long long)no * no * nb * 16 * kps;
-    pack_split_kernel<0><<<grid_for(ax_n), 256, 0, s>>>(axr, (__bf16 *)axr_s, (long long)no * nb * nb, kp, kps, 1.0 / sc[0]);
-    pack_split_kernel<0><<<grid_for(ax_n), 256, 0, s>>>(axc, (__bf16 *)axc_s, (long long)no * nb * nb, kp, kps, 1.0 / sc[0]);
-    pack_split_kernel<1><<<grid_for(bx_n), 256, 0, s>>>(bx, (__bf16 *)bx_s, (long long)no * no * nb, kp, kps, 1.0 / sc[1]);
+    if (form == 2) launch_pack_split<2>(axr, axc, bx, sl, no, nb, kp, kps, sc, s);
+    else if (form == 3) launch_pack_split<3>(axr, axc, bx, sl, no, nb, kp, kps, sc, s);
+    else launch_pack_split<4>(axr, axc, bx, sl, no, nb, kp, kps, sc, s);
     *wscale = sc[0] * sc[1];
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }

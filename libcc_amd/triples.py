@@ -17,7 +17,7 @@ import numpy as np
 
 from . import CcxError, lib
 
-FORMS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+FORMS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp8x4": 3, "bf16x4": 4}
 _dp = ctypes.POINTER(ctypes.c_double)
 
 

@@ -54,12 +54,16 @@ def run(inp: dict, mode: int, i: int = 0, j: int = 0, k: int = 0, lo: int = 0, h
 
 
 def triples(inp: dict, lo: int = 0, hi: int | None = None, degenerate: bool = True,
-            threads: int | None = None, packed: np.ndarray | None = None) -> dict:
-    """Driver over distinct tuples [lo,hi) (0-based ids) plus all degenerate tuples."""
+            threads: int | None = None, packed: np.ndarray | None = None, deg_pairs: int | None = None) -> dict:
+    """Driver over distinct tuples [lo,hi) (0-based ids) plus the degenerate tuples (i,j,j), (i,i,j)
+    of the first ``deg_pairs`` pairs i>j in the driver's order (all of them by default; none when
+    ``degenerate`` is False)."""
     no, nu = inp["no"], inp["nu"]
     if hi is None:
         hi = no * (no - 1) * (no - 2) // 6
-    o = run(inp, 1, k=1 if degenerate else 0, lo=lo, hi=hi, threads=threads, packed=packed)
+    npairs = no * (no - 1) // 2
+    k = 0 if not degenerate else (npairs if deg_pairs is None else int(deg_pairs))
+    o = run(inp, 1, k=k, lo=lo, hi=hi, threads=threads, packed=packed)
     return {"ets": o[0], "etd": o[1], "e": o[0] + o[1], "v1": o[2:2 + nu * no].reshape(nu, no, order="F"),
             "seconds": o[2 + nu * no]}
 

@@ -13,8 +13,9 @@
 !   in : mode, no, nu, i, j, k, lo, hi, eh(no), ep(nu), t1(no*nu), t2(nu*nu*no*no),
 !        vm(no*nu*no*no), voe(nu*nu*no*no), packed vvvo(nutr*nu*no)
 !   mode 1 : full driver over tuples [lo,hi) (0-based distinct-tuple ids, as
-!            DDCC_T_TASK numbers them) plus, when k > 0, all degenerate tuples;
-!            out: ets, etd, v1(nu*no), seconds spent in the tuple loops
+!            DDCC_T_TASK numbers them) plus the degenerate tuples (i,j,j), (i,i,j) of
+!            the first k pairs i>j in the driver's order (i outer; k >= no(no-1)/2:
+!            all of them); out: ets, etd, v1(nu*no), seconds spent in the tuple loops
 !   mode 2 : one distinct tuple (i,j,k) via DDCC_T_IJK_ACC; out: etd, v1, v3(nu^3)
 !   mode 3 : one (i,j,j) tuple via DDCC_T_IJJ_ACC; out: etd, v1, v3
 !   mode 4 : one (i,i,j) tuple via DDCC_T_IIJ_ACC; out: etd, v1, v3
@@ -94,8 +95,11 @@ case (1)
       call ddcc_t_ijk_acc(no, nu, ii, jj, kk, v1, t2, vm, v3, voe, eh, ep, ve_i, ve_j, ve_k)
    end do
    if (k .gt. 0) then
+      n = 0
       do ii = 1, no
          do jj = 1, ii-1
+            n = n + 1
+            if (n .gt. k) exit
             call getve(ii, ve_i)
             call tranmd_23(ve_i, nu, nu, nu, 1)
             call getve(jj, ve_j)

@@ -80,8 +80,8 @@ def test_unknown_form_rejected():
     ets, etd = ctypes.c_double(), ctypes.c_double()
     st = lib().ccx_energy(ctx._ctx, 7, ctypes.byref(ets), ctypes.byref(etd), None, 0)
     assert st != 0
-    with pytest.raises(KeyError):
-        ctx.energy("fp8x4")
+    with pytest.raises(ValueError):
+        ctx.energy("fp16")
     ctx.close()
 
 

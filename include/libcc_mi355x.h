@@ -39,6 +39,7 @@ extern "C" {
 #define CCX_ERR_INPUT 5    /* NaN/Inf in an input array                       */
 #define CCX_ERR_FORM 6     /* arithmetic form not available                   */
 #define CCX_ERR_IO 7       /* restart file missing, short or mis-framed       */
+#define CCX_ERR_SCRATCH 8  /* problem larger than the static scratch array of ccx_energy_static */
 
 /* arithmetic forms of the triples contraction; denominators, energies, v1 and all
  * reductions are FP64 in every form */
@@ -143,6 +144,22 @@ int ccx_gradient_form(ccx_ctx *ctx, int form, double *ets, double *etd, double *
 
 /* Set the stream (hipStream_t cast to void*) later calls use; NULL = default stream. */
 int ccx_set_stream(ccx_ctx *ctx, void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the FP64 solve of
+ * ccx_load_device + ccx_energy (unpacked VVVO) with every work buffer in one static device array
+ * of CCX_STATIC_SCRATCH_BYTES per GPU (held until the process ends) instead of allocated memory.
+ * All pointers are device pointers in the reference layouts; on return e[0] = etd, e[1] = ets and
+ * v1(nu,no) are being written on `stream` (hipStream_t cast to void*, NULL = default stream) and
+ * are ready when it is synchronized.  Results are bitwise those of ccx_energy.  Problems whose
+ * buffers exceed the array return CCX_ERR_SCRATCH before any device work (no=4, nu=87 needs about
+ * 58 MB; no=8, nu=120 does not fit).  NaN/Inf input returns CCX_ERR_INPUT as ccx_load_device does.
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one GPU (other host
+ * threads, other streams) race and must be serialized by the caller. */
+#define CCX_STATIC_SCRATCH_BYTES (64ull << 20)
+int ccx_energy_static(int64_t no, int64_t nu, const double *eh, const double *ep, const double *t1,
+                      const double *t2, const double *vm, const double *voe, const double *vvvo, double *e,
+                      double *v1, void *stream);
 
 /* ---------------------------------------------------------------- legacy Fortran ABI */
 

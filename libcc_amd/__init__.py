@@ -70,6 +70,8 @@ def lib() -> ctypes.CDLL:
     L.ccx_gradient_form.restype = ctypes.c_int
     L.ccx_set_stream.argtypes = [ctxp, vp]
     L.ccx_set_stream.restype = ctypes.c_int
+    L.ccx_energy_static.argtypes = [i64, i64] + [vp] * 9 + [vp]
+    L.ccx_energy_static.restype = ctypes.c_int
     pi64 = ctypes.POINTER(i64)
     L.common_cc_init_.argtypes = [pi64, pi64]
     L.common_cc_init_.restype = None

@@ -50,6 +50,7 @@ const char *ccx_error_string(int s)
     case CCX_ERR_INPUT: return "non-finite value in an input array";
     case CCX_ERR_FORM: return "arithmetic form not available";
     case CCX_ERR_IO: return "file missing, unreadable or not in the restart layout";
+    case CCX_ERR_SCRATCH: return "problem larger than the static scratch array";
     default: return "unknown status";
     }
 }

@@ -193,12 +193,17 @@ __device__ __forceinline__ unsigned char e4m3_of_int(int d)
 }
 
 // Slices of a slab in the fragment order of the form's MFMA (see run_phase_split).  One thread
-// per fragment group: the EL elements one lane holds for one K block of one row (AX: row = the
-// panel's outer index; BX: no outer index), K = KB kb + EL (lane >> 4) + e, inner index lane & 15.
-// The MFMA sums over every K of a block, so any placement of K inside a block that A and B share
-// gives the same products.  Output: [panel][outer][kb][slice][16-byte unit f][lane][16 B]; the FP8
-// form also writes one exponent word per group, [panel][outer][kb][lane].  Source: the FP64 slab,
-// AX [panel][outer/2][K/4][lane 64][2] or BX [panel][K][16 gamma].  inv_scale = 1 / sigma of the slab.
+// per group of EL elements of one row (AX: row = the panel's outer index; BX: no outer index),
+// inner index lane & 15.  Output: [panel][outer][kb][slice][16-byte unit f][lane][16 B].  The MFMA
+// sums over every K of a block, so any placement of K inside a block that A and B share gives
+// the same products:
+//   bf16 (EL 8): the group is lane's fragment, K = 32 kb + 8 (lane >> 4) + e;
+//   fp8 (EL 32): the group is scale block b = lane >> 4 of the 128-K block (K = 128 kb + 32 b + q,
+//     q < 32), which v_mfma_scale_f32_16x16x128_f8f6f4 reads as bytes 16 (b >> 1) + (q & 15) of
+//     lanes 16 (2 (b & 1) + (q >> 4)) + row and scales with the word of lane 16 b + row (the
+//     layout measured on the device); the group's exponent word goes to [panel][outer][kb][lane].
+// Source: the FP64 slab, AX [panel][outer/2][K/4][lane 64][2] or BX [panel][K][16 gamma].
+// inv_scale = 1 / sigma of the slab.
 template <int FK, int BX>
 __global__ void pack_split_kernel(const double *__restrict__ src, unsigned char *__restrict__ dst,
                                   int *__restrict__ dexp, long long npanel, int kp, int kps, double inv_scale)
@@ -241,12 +246,18 @@ __global__ void pack_split_kernel(const double *__restrict__ src, unsigned char 
         }
 #pragma unroll
         for (int e = 0; e < EL; ++e) {
-            const int f = (e * EB) / 16, byte = (e * EB) % 16;
+            int f = (e * EB) / 16, byte = (e * EB) % 16, ol = lane;
+            if constexpr (CF::MX) {   // element q = e of scale block b = lane >> 4
+                const int b = lane >> 4;
+                ol = 16 * (2 * (b & 1) + (e >> 4)) + inner;
+                f = b >> 1;
+                byte = e & 15;
+            }
             double t = v[e];
             if constexpr (CF::MX) t = ldexp(t, -gexp);
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                const long long o = (((rr * NS + sl) * F16 + f) * 64 + lane) * 16 + byte;
+                const long long o = (((rr * NS + sl) * F16 + f) * 64 + ol) * 16 + byte;
                 if constexpr (FK == 2) {          // per element: bf16 rounding of what is left
                     const __bf16 h = (__bf16)(float)t;
                     t -= (double)(float)h;

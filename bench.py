@@ -28,8 +28,13 @@ METRIC = "(T) time-to-solution & matrix-peak %; split-prec vs FP64 at |ΔE(T)|<1
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
-# executed MFMA FLOPs per algorithmic FLOP: the split BF16 form runs 6 slice products
-FORM_PEAK = {"fp64": (PEAK_FP64_TFLOPS, 1), "fp32": (PEAK_FP32_TFLOPS, 1), "bf16x3": (PEAK_BF16_TFLOPS, 6)}
+PEAK_FP8_TFLOPS = 5000.0
+# (peak of the matrix cores a form runs on, executed MFMA FLOPs per algorithmic FLOP): the split
+# forms run one MFMA per slice product (BF16 x 3: 6, BF16 x 4 and FP8 x 4: 10)
+FORM_PEAK = {"fp64": (PEAK_FP64_TFLOPS, 1), "fp32": (PEAK_FP32_TFLOPS, 1), "bf16x3": (PEAK_BF16_TFLOPS, 6),
+             "bf16x4": (PEAK_BF16_TFLOPS, 10), "fp8x4": (PEAK_FP8_TFLOPS, 10)}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32acc", "bf16x4": "bf16x4/f32acc", "fp8x4": "fp8e4m3x4/f32acc"}
+REDUCED_FORMS = ("fp32", "bf16x3", "bf16x4", "fp8x4")
 CONFIGS = {"c1": (4, 36, 1001), "c2": (4, 87, 1002), "c3": (8, 120, 1003), "c4": (24, 216, 1004),
            "c5": (30, 400, 1005)}
 
@@ -157,6 +162,89 @@ def cpu_baseline(inp_dev: dict, ctx, no: int, nu: int, budget_s: float, threads:
             "sample_parity": {"tuples": s_d + 2 * s_p, "etd_rel_diff": rel_de, "v1_rel_diff": rel_dv1}}
 
 
+def relabelled(inp: dict, no: int, nu: int, p_occ, p_vir) -> dict:
+    """The same problem with the occupied orbitals relabelled by p_occ and the virtual ones by
+    p_vir (1-D Fortran-ordered device buffers in and out): E(T) is invariant, so the FP64 |dE|
+    between the two is the form's own summation-order noise."""
+    import torch
+
+    po = torch.as_tensor(p_occ, device="cuda:0")
+    pv = torch.as_tensor(p_vir, device="cuda:0")
+
+    def arr(k, shape):   # Fortran-ordered buffer -> tensor indexed in reversed axis order
+        return inp[k].reshape(tuple(reversed(shape)))
+
+    out = {"eh": inp["eh"][po].contiguous(), "ep": inp["ep"][pv].contiguous()}
+    out["t1"] = arr("t1", (no, nu))[pv][:, po]
+    out["t2"] = arr("t2", (nu, nu, no, no))[po][:, po][:, :, pv][:, :, :, pv]
+    out["vm"] = arr("vm", (no, nu, no, no))[po][:, po][:, :, pv][:, :, :, po]
+    out["voe"] = arr("voe", (nu, nu, no, no))[po][:, po][:, :, pv][:, :, :, pv]
+    out["vvvo"] = arr("vvvo", (nu, nu, nu, no))[po][:, pv][:, :, pv][:, :, :, pv]
+    return {k: v.contiguous().reshape(-1) for k, v in out.items()}
+
+
+def reorder_floor(ctx, inp: dict, no: int, nu: int, e64: float) -> dict:
+    """|dE| of the FP64 form under a relabelling of the occupied and of the virtual orbitals."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(7)
+    res = {}
+    for which in ("occupied", "virtual"):
+        p_occ = rng.permutation(no) if which == "occupied" else np.arange(no)
+        p_vir = rng.permutation(nu) if which == "virtual" else np.arange(nu)
+        rel = relabelled(inp, no, nu, p_occ, p_vir)
+        ctx.load(rel)
+        res[which] = abs(ctx.energy("fp64")["e"] - e64)
+        del rel
+        torch.cuda.empty_cache()
+    ctx.load(inp)
+    return {"abs_de_relabel_occupied": res["occupied"], "abs_de_relabel_virtual": res["virtual"],
+            "rel": max(res.values()) / abs(e64)}
+
+
+def legacy_tuple_times(inp_dev: dict, no: int, nu: int, ncall: int) -> dict:
+    """Per-call time of the reference-ABI per-tuple entry ddcc_t_ijk_acc_ (host arrays, as a
+    Fortran driver passes them, src/triples-ijk/ddcc_t_ijk_big.f:1-21) over the first ncall + 1
+    distinct tuples; the first call, which creates the per-tuple workspace, is reported apart."""
+    import ctypes
+
+    import numpy as np
+
+    from libcc_amd import lib
+
+    L = lib()
+    i64 = ctypes.c_int64
+
+    def I(v):
+        return ctypes.byref(i64(v))
+
+    def D(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    host = {k: np.asfortranarray(inp_dev[k].cpu().numpy()) for k in ("eh", "ep", "t2", "vm", "voe")}
+    vv = inp_dev["vvvo"].reshape(no, nu, nu, nu)     # [x][p][q][d] = VVVO(d,q,p,x)
+    tups = [(i, j, k) for i in range(no) for j in range(i) for k in range(j)][:ncall + 1]
+    ve = {}
+    for t in tups:
+        for x in t:
+            if x not in ve:   # VE_x(d,p,q) = VVVO(d,q,p,x): Fortran buffer = C-order [q][p][d]
+                ve[x] = np.ascontiguousarray(vv[x].permute(1, 0, 2).cpu().numpy()).reshape(-1)
+    v1 = np.zeros(nu * no)
+    v3 = np.zeros(nu ** 3)
+    L.common_cc_init_(I(no), I(nu))
+    times = []
+    for (i, j, k) in tups:
+        t0 = time.perf_counter()
+        L.ddcc_t_ijk_acc_(I(no), I(nu), I(i + 1), I(j + 1), I(k + 1), D(v1), D(host["t2"]), D(host["vm"]), D(v3),
+                          D(host["voe"]), D(host["eh"]), D(host["ep"]), D(ve[i]), D(ve[j]), D(ve[k]))
+        times.append(time.perf_counter() - t0)
+        if L.ccx_last_status() != 0:
+            raise RuntimeError(f"ddcc_t_ijk_acc_ status {L.ccx_last_status()}")
+    return {"entry": "ddcc_t_ijk_acc_", "no": no, "nu": nu, "first_call_ms": times[0] * 1e3,
+            "calls": len(times) - 1, "ms_per_call": float(np.mean(times[1:])) * 1e3 if len(times) > 1 else None}
+
+
 def reduced_adjoint(ctx, g64: dict, form: str) -> dict:
     """The adjoint with W recomputed in a reduced form (fp32, bf16x3): time and, per gradient
     array, the error against the FP64 adjoint in the relative Frobenius norm."""
@@ -185,6 +273,8 @@ def main() -> None:
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run one adjoint (gradient) evaluation and print its timing (with --form fp32 or "
                          "bf16x3 also that form's adjoint and its gradient error against FP64)")
+    ap.add_argument("--legacy-tuples", type=int, default=0, metavar="N",
+                    help="only time N calls of the per-tuple entry ddcc_t_ijk_acc_ on host arrays")
     ap.add_argument("--form", default="fp64", choices=sorted(FORM_PEAK),
                     help="arithmetic form of the timed steps (the headline metric is fp64)")
     ap.add_argument("--no-extra", action="store_true",
@@ -218,6 +308,10 @@ def main() -> None:
     log(f"generating no={no} nu={nu} seed={seed} on cuda:0")
     inp = synth.make_inputs_torch(no, nu, seed, "cuda:0")
     torch.cuda.synchronize()
+    if args.legacy_tuples:
+        log(f"legacy per-tuple entry, {args.legacy_tuples} calls")
+        print(json.dumps(legacy_tuple_times(inp, no, nu, args.legacy_tuples)), flush=True)
+        return
     ctx = TriplesContext(no, nu)
     if args.adjoint_only:
         ctx.load(inp)
@@ -267,7 +361,7 @@ def main() -> None:
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32acc"}[args.form],
+        "dtype": DTYPE[args.form],
         "data": "synthetic",
         "config": {"workload": f"closed-shell (T) E(T)=E[T]+E_ST and v1, no={no} nu={nu}, "
                                f"{no * (no - 1) * (no - 2) // 6} distinct + {no * (no - 1)} degenerate tuples, "
@@ -287,7 +381,7 @@ def main() -> None:
         out["forms"] = {"fp64": {"kernel_ms": kernel_s * 1e3, "e_t": r["e"], "abs_de": 0.0,
                                  "frac_of_peak": achieved / PEAK_FP64_TFLOPS,
                                  "peak_tflops": PEAK_FP64_TFLOPS}}
-        for form in ("fp32", "bf16x3"):
+        for form in REDUCED_FORMS:
             peak, mult = FORM_PEAK[form]
             log(form)
             ctx.energy(form)
@@ -298,6 +392,9 @@ def main() -> None:
                                   "speedup_vs_fp64": kernel_s / kf,
                                   "frac_of_peak": mult * flops / kf / 1e12 / peak, "peak_tflops": peak,
                                   "executed_flops_per_algorithmic": mult}
+        # the FP64 form's own summation-order noise, the floor any |dE| is read against
+        log("fp64 reorder floor")
+        out["forms"]["fp64"]["reorder_floor"] = reorder_floor(ctx, inp, no, nu, r["e"])
         # the adjoint: dE/d(T1, T2, VM, VOE, VVVO) with per-item recomputation of W
         log("adjoint")
         g = ctx.gradient(device=True)

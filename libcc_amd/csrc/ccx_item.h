@@ -632,25 +632,35 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
             }
         }
     };
-    // A ring step is one operand slot and BS of the wave's batches: both for bf16, one at a time
-    // for the 32-byte fp8 fragments (the registers of a two-batch step would not fit next to the
-    // accumulators).
+    // A ring step is SPS operand slots and BS of the wave's batches: one slot of both batches for
+    // bf16, one slot and one batch at a time for the 32-byte fp8 fragments.  Two bf16 slots per
+    // step (a longer prefetch distance) measured 4 % (BF16x3) and 12 % (BF16x4) slower at no=24,
+    // nu=216: the larger ring costs registers, and the sweep is not waiting on its loads.
     typedef split_frag_t<FK> frag_t;
     constexpr int BS = CF::FMT == 0 ? BPW : 1, NH = BPW / BS;
-    frag_t ab[2][BS][NS];
-    int ae[2][BS];
-    auto a_load = [&](int s, int h, int kb, frag_t (&d)[BS][NS], int (&de)[BS]) __attribute__((always_inline)) {
+    constexpr int SPS = 1;
+    constexpr int NG = (NSL + SPS - 1) / SPS;          // slot groups per K block
+    frag_t ab[2][SPS][BS][NS];
+    int ae[2][SPS][BS];
+    auto a_load = [&](auto Gc, int h, int kb, frag_t (&d)[SPS][BS][NS], int (&de)[SPS][BS]) __attribute__((always_inline)) {
+        constexpr int gi = decltype(Gc)::value;
+        static_for<SPS>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value, j = gi * SPS + u;
+            if constexpr (j < NSL) {
+                constexpr int s = nth_bit(SM, j);
 #pragma unroll
-        for (int b = 0; b < BS; ++b) {
-            const int bb = h * BS + b;
+                for (int b = 0; b < BS; ++b) {
+                    const int bb = h * BS + b;
 #pragma unroll
-            for (int sl = 0; sl < NS; ++sl)
-                d[b][sl] = split_frag_load<FK>(ap[s], (((long long)bb * nkb + kb) * CH + sl * F16) * 64 + lane);
-            if constexpr (CF::MX) de[b] = aep[s][((long long)bb * nkb + kb) * 64 + lane];
-        }
+                    for (int sl = 0; sl < NS; ++sl)
+                        d[u][b][sl] = split_frag_load<FK>(ap[s], (((long long)bb * nkb + kb) * CH + sl * F16) * 64 + lane);
+                    if constexpr (CF::MX) de[u][b] = aep[s][((long long)bb * nkb + kb) * 64 + lane];
+                }
+            }
+        });
     };
-    auto compute = [&](auto Sc, auto Hc, const frag_t (&a)[BS][NS], const int (&ea)[BS], int buf)
-                       __attribute__((always_inline)) {
+    auto compute_slot = [&](auto Sc, auto Hc, const frag_t (&a)[BS][NS], const int (&ea)[BS], int buf)
+                            __attribute__((always_inline)) {
         constexpr int s = decltype(Sc)::value, h = decltype(Hc)::value;
         constexpr SlotTerms ST = slot_terms(PHASE, MODE, s);
         int eb[ST.n > 0 ? ST.n : 1];
@@ -688,13 +698,26 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
             });
         });
     };
+    auto compute = [&](auto Gc, auto Hc, const frag_t (&a)[SPS][BS][NS], const int (&ea)[SPS][BS], int buf)
+                       __attribute__((always_inline)) {
+        constexpr int gi = decltype(Gc)::value;
+        static_for<SPS>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value, j = gi * SPS + u;
+            if constexpr (j < NSL) compute_slot(std::integral_constant<int, nth_bit(SM, j)>{}, Hc, a[u], ea[u], buf);
+        });
+    };
 
-    // steps of one K block: (slot j, batch group h), j major; NST = NSL * NH per block
-    constexpr int NST = NSL * NH;
+    // steps of one K block: (slot group g, batch group h), g major; NST = NG * NH per block
+    constexpr int NST = NG * NH;
+    // s_waitcnt immediate (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+    // that waits until at most NPF vector-memory loads are outstanding, nothing else
+    constexpr int NPF = (NSL < SPS ? NSL : SPS) * BS * (NS * F16 + (CF::MX ? 1 : 0));
+    static_assert(NPF < 64, "vmcnt range");
+    constexpr int WAIT_DMA = (NPF & 15) | ((NPF >> 4) << 14) | (7 << 4) | (15 << 8);
     bx_load(0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    a_load(nth_bit(SM, 0), 0, 0, ab[0], ae[0]);
+    a_load(std::integral_constant<int, 0>{}, 0, 0, ab[0], ae[0]);
     for (int kb = 0; kb < nkb; kb += 2) {
         static_for<2>([&](auto HB) __attribute__((always_inline)) {
             constexpr int hb = decltype(HB)::value;
@@ -704,15 +727,18 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
             static_for<NST>([&](auto J) __attribute__((always_inline)) {
                 constexpr int j = decltype(J)::value;
                 constexpr int t = hb * NST + j;
-                constexpr int s = nth_bit(SM, j / NH), h = j % NH;
-                constexpr int sn = nth_bit(SM, ((j + 1) % NST) / NH), hn = ((j + 1) % NST) % NH;
+                constexpr int gi = j / NH, h = j % NH;
+                constexpr int gn = ((j + 1) % NST) / NH, hn = ((j + 1) % NST) % NH;
                 const int kn = j + 1 < NST ? kbb : (kbb + 1 < nkb ? kbb + 1 : nkb - 1);
-                a_load(sn, hn, kn, ab[(t + 1) % 2], ae[(t + 1) % 2]);
+                a_load(std::integral_constant<int, gn>{}, hn, kn, ab[(t + 1) % 2], ae[(t + 1) % 2]);
                 __builtin_amdgcn_sched_barrier(0);
-                compute(std::integral_constant<int, s>{}, std::integral_constant<int, h>{}, ab[t % 2], ae[t % 2], hb);
+                compute(std::integral_constant<int, gi>{}, std::integral_constant<int, h>{}, ab[t % 2], ae[t % 2], hb);
                 __builtin_amdgcn_sched_barrier(0);
             });
-            __builtin_amdgcn_s_waitcnt(0);
+            // the staged block must have landed before the barrier publishes it; the AX prefetch
+            // of the next step (the NPF most recent loads; loads retire in issue order) may stay
+            // in flight across the barrier
+            __builtin_amdgcn_s_waitcnt(WAIT_DMA);
             __syncthreads();
         });
     }

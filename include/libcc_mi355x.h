@@ -120,7 +120,9 @@ int ccx_energy_abc(ccx_ctx *ctx, double *etd);
  * persistent kernel only, and around the whole call). */
 int ccx_last_timing(const ccx_ctx *ctx, double *kernel_ms, double *total_ms);
 
-/* Work items and work-item FLOPs of the persistent launch (for roofline accounting). */
+/* Work items and the MFMA FLOPs the FP64 form's persistent launch executes (16-padded
+ * virtual blocks, less the sweeps of waves whose batches are all padding), for roofline
+ * accounting. */
 int ccx_work_stats(const ccx_ctx *ctx, int64_t *items, double *executed_flops);
 
 /* Adjoint of E(T) = etd + ets with respect to every input array (FP64), by re-running the

@@ -619,12 +619,25 @@ int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
     const int64_t nitems = c->ntup * c->nblk;
     if (items) *items = nitems;
     /* per item: 6 MFMA terms x 16^3 outputs x kp MACs per accumulated ordering; orderings:
-     * 6 (distinct tuple, A<B<C), 3 (two equal blocks or a degenerate tuple), 1 (A=B=C) */
+     * 6 (distinct tuple, A<B<C), 3 (two equal blocks or a degenerate tuple), 1 (A=B=C).
+     * FP64 form: of the 8 waves (2 batches each) a phase runs only those whose batches reach
+     * a virtual index of the phase's batch block (C in phase 0, B in phases 1 and 2, with a
+     * ragged C of three distinct blocks moved to role B). */
     double units = 0.0;
     for (int64_t t = 0; t < c->ntup; ++t)
         for (int64_t q = 0; q < c->nblk; ++q) {
             const int kind = c->h_tuples[4 * t + 3], mult = c->h_blocks[4 * q + 3];
-            units += kind != 1 ? 3.0 : mult == 1 ? 6.0 : mult == 2 ? 3.0 : 1.0;
+            int bb = c->h_blocks[4 * q + 1], bc = c->h_blocks[4 * q + 2];
+            if (mult == 1 && c->nu - 16 * bc < 16) { const int x = bb; bb = bc; bc = x; }
+            double fr[2];
+            const int blk[2] = {bc, bb};
+            for (int r = 0; r < 2; ++r) {
+                int real = (int)(c->nu - 16 * blk[r]);
+                if (real > 16) real = 16;
+                fr[r] = (double)((real + 1) / 2) / 8.0;
+            }
+            const double ord = kind != 1 ? 3.0 : mult == 1 ? 6.0 : mult == 2 ? 3.0 : 1.0;
+            units += ord * (fr[0] + 2.0 * fr[1]) / 3.0;
         }
     if (executed_flops) *executed_flops = units * 6.0 * 4096.0 * 2.0 * (double)c->kp;
     return CCX_OK;

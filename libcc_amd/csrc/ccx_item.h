@@ -251,17 +251,30 @@ __device__ __forceinline__ void mfma_step(const Frags<T> &f, typename Mfma<T>::a
         });
 }
 
+// Whether this wave's two batches reach a virtual index of the phase's batch block (C in
+// phase 0, B in phases 1 and 2).  A wave whose batches lie past the last virtual index (the
+// ragged last block) has only zero outputs in that phase and skips its sweep; its SIMD then
+// runs the other wave's MFMAs alone.
+template <int PHASE>
+__device__ __forceinline__ bool phase_active(const TriplesArgs &g, const ItemCtx &it, int wave)
+{
+    const int bblk = PHASE == 0 ? it.blk[2] : it.blk[1];
+    return wave * BPW < g.nu - bblk * TB;
+}
+
 // One K sweep.  Register double buffering: fragments of step kb+1 load while step kb runs.
 template <int PHASE, int MODE, class T>
 __device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &it,
                                           typename Mfma<T>::acc_t (&acc)[6][BPW], int wave, int lane)
 {
+    if (!phase_active<PHASE>(g, it, wave)) return;
     PhasePtrs<T> pp;
     phase_ptrs<PHASE, T>(g, it, wave, pp);
     // Ring of NBUF register buffers: the loads of step kb+NBUF-1 are in flight while step kb
-    // computes.  FP64 uses 2 (one step covers the load latency at the FP64 MFMA rate); FP32
-    // fragments take half the registers and its MFMAs run twice as fast, so it uses 4.
-    constexpr int NBUF = std::is_same<T, double>::value ? 2 : 4;
+    // computes.  FP64 uses 2: one step covers the load latency at the FP64 MFMA rate.  Three
+    // or four buffers for the items with fewer orderings, and the first fragments of the next
+    // phase requested before the end of the current one, measured the same at no=24, nu=216.
+    constexpr int NBUF = 2;
     const int nkb = g.kp / KS;
     const int last = nkb - 1;
     Frags<T> f[NBUF];
@@ -285,7 +298,6 @@ __device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &i
             __builtin_amdgcn_sched_barrier(0);
         });
     }
-    // the remaining nkb - kb (< NBUF) steps are in f[0 ..]
     static_for<NBUF - 1>([&](auto J) __attribute__((always_inline)) {
         constexpr int jn = decltype(J)::value;
         if (kb + jn < nkb) mfma_step<PHASE, MODE, T>(f[jn], acc);

@@ -295,7 +295,7 @@ namespace {
 // spills in scratch memory (0.06 TB of writes per launch at no=24, nu=216).
 template <int MODE, class T, int FK>
 __device__ __forceinline__ void do_item(const TriplesArgs &g, double *lds, double (*red)[112], const ItemCtx &it, long long item,
-                        int tkind, int mult, int wave, int lane, int tid)
+                        int tkind, int mult, bool swap_bc, int wave, int lane, int tid)
 {
     typename Mfma<T>::acc_t acc[6][BPW];
     f4 acc1[FK == 4 ? 6 : 1][BPW];   // two-accumulator split form: the levels above 0
@@ -413,6 +413,7 @@ __device__ __forceinline__ void do_item(const TriplesArgs &g, double *lds, doubl
     }
     __syncthreads();
     double *vp = g.v1_part + item * 144;
+    const int slot_b = swap_bc ? 96 : 48, slot_c = swap_bc ? 48 : 96;   // table role order
     if (tid < 48) {
         double s = 0.0;
 #pragma unroll
@@ -420,13 +421,13 @@ __device__ __forceinline__ void do_item(const TriplesArgs &g, double *lds, doubl
         vp[tid] = s * wgt;
     } else if (tid < 96) {
         const int r = tid - 48, row = r / 3, p = r % 3;
-        vp[48 + r] = red[row / BPW][48 + (row % BPW) * 3 + p] * wgt;
+        vp[slot_b + r] = red[row / BPW][48 + (row % BPW) * 3 + p] * wgt;
     } else if (tid < 144) {
         const int r = tid - 96;
         double s = 0.0;
 #pragma unroll
         for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][54 + r];
-        vp[96 + r] = s * wgt;
+        vp[slot_c + r] = s * wgt;
     } else if (tid == 144) {
         double s = 0.0;
 #pragma unroll
@@ -465,13 +466,26 @@ __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
         const int t = __builtin_amdgcn_readfirstlane((int)(pos % g.ntup));
         const long long item = (long long)t * g.nblk + q;   // storage index
         int tkind, mult;
-        const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
-        if (tkind == 2) do_item<M_IJJ, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (tkind == 3) do_item<M_IIJ, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (mult == 1) do_item<M_FULL, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (mult == 6) do_item<M_AAA, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
-        else do_item<M_ACC, T, FK>(g, lds, red, it, item, tkind, mult, wave, lane, tid);
+        ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
+        // FP64 form, three distinct blocks whose last one is ragged (fewer than 16 virtual
+        // indices): that block takes role B, the batch axis of two of the three K phases (role C
+        // is the batch axis of one), where the waves holding only padding skip the sweep.  The
+        // orbit's cube and its six orderings are the same sums in either role order; the v1
+        // partials are stored back in the block table's role order.  The other forms keep the
+        // table's roles (their sweeps stage BX with all waves and do not skip), so their W
+        // is rounded exactly as the adjoint's recomputation of it.
+        const bool swap_bc = FK == 0 && mult == 1 && g.nu - it.blk[2] * TB < TB;
+        if (swap_bc) {
+            const int b = it.blk[1];
+            it.blk[1] = it.blk[2];
+            it.blk[2] = b;
+        }
+        if (tkind == 2) do_item<M_IJJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        else if (tkind == 3) do_item<M_IIJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        else if (mult == 1) do_item<M_FULL, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        else if (mult == 6) do_item<M_AAA, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        else do_item<M_ACC, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
     }
 }
 

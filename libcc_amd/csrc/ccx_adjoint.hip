@@ -83,8 +83,8 @@ __device__ __forceinline__ double staged_w(const double *ws, int p, int bb, int 
 }
 
 __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112], const ItemCtx &it,
-                             long long item, int tl, int tkind, int mult, const double *ws, int wave, int lane,
-                             int tid)
+                             long long item, int tl, int tkind, int mult, bool swap_bc, const double *ws, int wave,
+                             int lane, int tid)
 {
     const TriplesArgs &g = ga.f;
     const int nb = g.nb;
@@ -180,6 +180,7 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
     }
     __syncthreads();
     double *vp = g.v1_part + item * 144;
+    const int slot_b = swap_bc ? 96 : 48, slot_c = swap_bc ? 48 : 96;   // table role order
     if (tid < 48) {
         double s = 0.0;
 #pragma unroll
@@ -187,13 +188,13 @@ __device__ void adj_epilogue(const AdjArgs &ga, double *lds, double (*red)[112],
         vp[tid] = s * wgt;
     } else if (tid < 96) {
         const int r = tid - 48, row = r / 3, p = r % 3;
-        vp[48 + r] = red[row / BPW][48 + (row % BPW) * 3 + p] * wgt;
+        vp[slot_b + r] = red[row / BPW][48 + (row % BPW) * 3 + p] * wgt;
     } else if (tid < 144) {
         const int r = tid - 96;
         double s = 0.0;
 #pragma unroll
         for (int w2 = 0; w2 < NWAVE; ++w2) s += red[w2][54 + r];
-        vp[96 + r] = s * wgt;
+        vp[slot_c + r] = s * wgt;
     } else if (tid == 144) {
         double s = 0.0;
 #pragma unroll
@@ -376,7 +377,8 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
         const int t = ga.t0 + tl;
         const long long item = (long long)t * g.nblk + q;
         int tkind, mult;
-        const ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
+        ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
+        const bool swap_bc = ragged_block_to_b(g, it, mult);   // as the forward kernel does
         double *ws = ga.wstage + (long long)blockIdx.x * (6 * BPW * 4 * NTHR);
         if constexpr (FK != 0) {
             f4 acc[6][BPW];
@@ -422,7 +424,7 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) ws[((p * BPW + bb) * 4 + v) * NTHR + tid] = acc[p][bb][v];
         }
-        adj_epilogue(ga, lds, red, it, item, tl, tkind, mult, ws, wave, lane, tid);
+        adj_epilogue(ga, lds, red, it, item, tl, tkind, mult, swap_bc, ws, wave, lane, tid);
     }
 }
 

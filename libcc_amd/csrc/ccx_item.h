@@ -262,6 +262,21 @@ __device__ __forceinline__ bool phase_active(const TriplesArgs &g, const ItemCtx
     return wave * BPW < g.nu - bblk * TB;
 }
 
+// Three distinct blocks whose last one is ragged (fewer than 16 virtual indices): that block
+// takes role B, the batch axis of two of the three K phases (role C is the batch axis of one),
+// where the waves holding only padding skip the sweep.  The orbit's cube and its six
+// orderings are the same sums in either role order, and every store of an item is placed by
+// the item's own roles except the v1 partials, which the caller stores back in the block
+// table's role order when this returns true.
+__device__ __forceinline__ bool ragged_block_to_b(const TriplesArgs &g, ItemCtx &it, int mult)
+{
+    if (mult != 1 || g.nu - it.blk[2] * TB >= TB) return false;
+    const int b = it.blk[1];
+    it.blk[1] = it.blk[2];
+    it.blk[2] = b;
+    return true;
+}
+
 // One K sweep.  Register double buffering: fragments of step kb+1 load while step kb runs.
 template <int PHASE, int MODE, class T>
 __device__ __forceinline__ void run_phase(const TriplesArgs &g, const ItemCtx &it,
@@ -729,6 +744,19 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
     bx_load(0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    if (!phase_active<PHASE>(g, it, wave)) {
+        // only padding batches: stage this wave's BX chunks and meet every barrier
+        for (int kb = 0; kb < nkb; kb += 2) {
+            static_for<2>([&](auto HB) __attribute__((always_inline)) {
+                constexpr int hb = decltype(HB)::value;
+                const int kbb = kb + hb;
+                bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1, 1 - hb);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            });
+        }
+        return;
+    }
     a_load(std::integral_constant<int, 0>{}, 0, 0, ab[0], ae[0]);
     for (int kb = 0; kb < nkb; kb += 2) {
         static_for<2>([&](auto HB) __attribute__((always_inline)) {
@@ -864,6 +892,19 @@ __device__ __forceinline__ void run_phase_f32v(const TriplesArgs &g, const ItemC
     bx_load(0);
     bx_store(0);
     __syncthreads();
+    if (!phase_active<PHASE>(g, it, wave)) {
+        // only padding batches: stage this wave's BX chunks and meet every barrier
+        for (int kb = 0; kb < nkb; kb += 3) {
+            static_for<3>([&](auto HB) __attribute__((always_inline)) {
+                constexpr int hb = decltype(HB)::value;
+                const int kbb = kb + hb;
+                bx_load(kbb + 1 < nkb ? kbb + 1 : nkb - 1);
+                bx_store((hb + 1) % 3);
+                __syncthreads();
+            });
+        }
+        return;
+    }
     // steps t = block * NSL + slot index; the ring holds steps t, t+1, t+2
     a_load(nth_bit(SM, 0), 0, ab[0]);
     a_load(nth_bit(SM, 1 % NSL), 1 / NSL < nkb ? 1 / NSL : nkb - 1, ab[1]);

@@ -467,19 +467,7 @@ __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
         const long long item = (long long)t * g.nblk + q;   // storage index
         int tkind, mult;
         ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
-        // FP64 form, three distinct blocks whose last one is ragged (fewer than 16 virtual
-        // indices): that block takes role B, the batch axis of two of the three K phases (role C
-        // is the batch axis of one), where the waves holding only padding skip the sweep.  The
-        // orbit's cube and its six orderings are the same sums in either role order; the v1
-        // partials are stored back in the block table's role order.  The other forms keep the
-        // table's roles (their sweeps stage BX with all waves and do not skip), so their W
-        // is rounded exactly as the adjoint's recomputation of it.
-        const bool swap_bc = FK == 0 && mult == 1 && g.nu - it.blk[2] * TB < TB;
-        if (swap_bc) {
-            const int b = it.blk[1];
-            it.blk[1] = it.blk[2];
-            it.blk[2] = b;
-        }
+        const bool swap_bc = ragged_block_to_b(g, it, mult);
         if (tkind == 2) do_item<M_IJJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
         else if (tkind == 3) do_item<M_IIJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
         else if (mult == 1) do_item<M_FULL, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);

@@ -384,6 +384,7 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
             f4 acc[6][BPW];
             if constexpr (FK == 2) {
                 f4 acc1[1][BPW];   // unused: the BF16 x 3 form has one accumulator
+                static_assert(!pair_modes(2), "the forward kernel's modes");
                 if (tkind == 2) item_w_split<M_IJJ, 2>(g, lds, it, acc, acc1, wave, lane);
                 else if (tkind == 3) item_w_split<M_IIJ, 2>(g, lds, it, acc, acc1, wave, lane);
                 else if (mult == 1) item_w_split<M_FULL, 2>(g, lds, it, acc, acc1, wave, lane);
@@ -391,6 +392,7 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
                 else if (it.blk[0] == it.blk[1]) item_w_split<M_AAC, 2>(g, lds, it, acc, acc1, wave, lane);
                 else item_w_split<M_ACC, 2>(g, lds, it, acc, acc1, wave, lane);
             } else {
+                static_assert(!pair_modes(1), "the forward kernel's modes");
                 if (tkind == 2) item_w_f32v<M_IJJ>(g, lds, it, acc, wave, lane);
                 else if (tkind == 3) item_w_f32v<M_IIJ>(g, lds, it, acc, wave, lane);
                 else if (mult == 1) item_w_f32v<M_FULL>(g, lds, it, acc, wave, lane);
@@ -411,12 +413,10 @@ __global__ void __launch_bounds__(NTHR, 1) adj_item_kernel(AdjArgs ga)
             __syncthreads();
         } else {
             d4 acc[6][BPW];
-            if (tkind == 2) item_w<M_IJJ, double>(g, lds, it, acc, wave, lane);
-            else if (tkind == 3) item_w<M_IIJ, double>(g, lds, it, acc, wave, lane);
-            else if (mult == 1) item_w<M_FULL, double>(g, lds, it, acc, wave, lane);
-            else if (mult == 6) item_w<M_AAA, double>(g, lds, it, acc, wave, lane);
-            else if (it.blk[0] == it.blk[1]) item_w<M_AAC, double>(g, lds, it, acc, wave, lane);
-            else item_w<M_ACC, double>(g, lds, it, acc, wave, lane);
+            static_assert(pair_modes(0), "the forward kernel's modes");
+            dispatch_mode(tkind, mult, it, [&](auto M) __attribute__((always_inline)) {
+                item_w<decltype(M)::value, double>(g, lds, it, acc, wave, lane);
+            });
 #pragma unroll
             for (int p = 0; p < 6; ++p)
 #pragma unroll

@@ -619,7 +619,8 @@ int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
     const int64_t nitems = c->ntup * c->nblk;
     if (items) *items = nitems;
     /* per item: 6 MFMA terms x 16^3 outputs x kp MACs per accumulated ordering; orderings:
-     * 6 (distinct tuple, A<B<C), 3 (two equal blocks or a degenerate tuple), 1 (A=B=C).
+     * 6 (distinct tuple, A<B<C), 3 (two equal blocks, or a degenerate tuple on distinct
+     * blocks), 2 (a degenerate tuple on two equal blocks), 1 (A=B=C).
      * FP64 form: of the 8 waves (2 batches each) a phase runs only those whose batches reach
      * a virtual index of the phase's batch block (C in phase 0, B in phases 1 and 2, with a
      * ragged C of three distinct blocks moved to role B). */
@@ -636,7 +637,7 @@ int ccx_work_stats(const ccx_ctx *c, int64_t *items, double *executed_flops)
                 if (real > 16) real = 16;
                 fr[r] = (double)((real + 1) / 2) / 8.0;
             }
-            const double ord = kind != 1 ? 3.0 : mult == 1 ? 6.0 : mult == 2 ? 3.0 : 1.0;
+            const double ord = mult == 6 ? 1.0 : kind != 1 ? (mult == 1 ? 3.0 : 2.0) : mult == 1 ? 6.0 : 3.0;
             units += ord * (fr[0] + 2.0 * fr[1]) / 3.0;
         }
     if (executed_flops) *executed_flops = units * 6.0 * 4096.0 * 2.0 * (double)c->kp;

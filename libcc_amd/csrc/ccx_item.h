@@ -68,13 +68,18 @@ __host__ __device__ constexpr int wsrc_comp(int p, int s, int t)
     return wsrc(p, pi);
 }
 
-// Item modes: which orderings are accumulated.
-enum Mode { M_FULL = 0, M_AAC = 1, M_ACC = 2, M_AAA = 3, M_IJJ = 4, M_IIJ = 5 };
+// Item modes: which orderings are accumulated.  Equal blocks of the orbit (AAC: A = B, ACC:
+// B = C, AAA) relate orderings by transposes inside the cube, equal occupied indices of the tuple
+// ((i,j,j), (i,i,j)) pointwise; a degenerate tuple on an orbit with two equal blocks uses both.
+enum Mode { M_FULL = 0, M_AAC = 1, M_ACC = 2, M_AAA = 3, M_IJJ = 4, M_IIJ = 5,
+            M_IJJ_AAC = 6, M_IJJ_ACC = 7, M_IIJ_AAC = 8, M_IIJ_ACC = 9 };
 
 __host__ __device__ constexpr int mode_mask(int mode)
 {
     return mode == M_FULL ? 0x3f : mode == M_AAC ? 0x0b /*0,1,3*/ : mode == M_ACC ? 0x15 /*0,2,4*/
-         : mode == M_AAA ? 0x01 : mode == M_IJJ ? 0x0d /*0,2,3*/ : 0x13 /*0,1,4*/;
+         : mode == M_AAA ? 0x01 : mode == M_IJJ ? 0x0d /*0,2,3*/ : mode == M_IIJ ? 0x13 /*0,1,4*/
+         : mode == M_IJJ_AAC ? 0x09 /*0,3*/ : mode == M_IJJ_ACC ? 0x05 /*0,2*/
+         : mode == M_IIJ_AAC ? 0x03 /*0,1*/ : 0x11 /*0,4*/;
 }
 
 // operand slots a phase needs for the computed orderings
@@ -422,6 +427,14 @@ __device__ __forceinline__ void complete_orderings(double *lds, typename Mfma<T>
                     acc[1][bb][v] = lds[0 * CUBE + cidx(e[0], e[2], e[1])];
                     acc[3][bb][v] = lds[1 * CUBE + cidx(e[0], e[2], e[1])];
                     acc[5][bb][v] = lds[2 * CUBE + cidx(e[0], e[2], e[1])];
+                } else if constexpr (MODE == M_IJJ_AAC) {   // cubes w0, w3: w2 = w0(b,a,c)
+                    acc[2][bb][v] = lds[0 * CUBE + cidx(e[1], e[0], e[2])];
+                } else if constexpr (MODE == M_IJJ_ACC) {   // cubes w0, w2: w3 = w2(a,c,b)
+                    acc[3][bb][v] = lds[1 * CUBE + cidx(e[0], e[2], e[1])];
+                } else if constexpr (MODE == M_IIJ_AAC) {   // cubes w0, w1: w4 = w1(b,a,c)
+                    acc[4][bb][v] = lds[1 * CUBE + cidx(e[1], e[0], e[2])];
+                } else if constexpr (MODE == M_IIJ_ACC) {   // cubes w0, w4: w1 = w0(a,c,b)
+                    acc[1][bb][v] = lds[0 * CUBE + cidx(e[0], e[2], e[1])];
                 } else {                               // A=B=C: w_P(e) = w_0(P^-1 . e)
                     static_for<5>([&](auto Q) __attribute__((always_inline)) {
                         constexpr int p = decltype(Q)::value + 1;
@@ -432,7 +445,50 @@ __device__ __forceinline__ void complete_orderings(double *lds, typename Mfma<T>
                     });
                 }
             }
+        // then the pointwise relations of the degenerate tuple
+        if constexpr (MODE == M_IJJ_AAC || MODE == M_IJJ_ACC) {   // (i,j,j): w1 = w0, w4 = w2, w5 = w3
+#pragma unroll
+            for (int bb = 0; bb < BPW; ++bb) { acc[1][bb] = acc[0][bb]; acc[4][bb] = acc[2][bb]; acc[5][bb] = acc[3][bb]; }
+        } else if constexpr (MODE == M_IIJ_AAC || MODE == M_IIJ_ACC) {   // (i,i,j): w2 = w0, w3 = w1, w5 = w4
+#pragma unroll
+            for (int bb = 0; bb < BPW; ++bb) { acc[2][bb] = acc[0][bb]; acc[3][bb] = acc[1][bb]; acc[5][bb] = acc[4][bb]; }
+        }
         __syncthreads();
+    }
+}
+
+// Whether form FK (0 FP64, 1 FP32, 2-4 split) runs a degenerate tuple on an orbit with equal
+// blocks in the modes that use both symmetries (two orderings, or one for three equal blocks).
+// Measured at no=24, nu=216: FP64 1143 -> 1132 ms, BF16x4 1375 -> 1359, FP8x4 4116 -> 4057; FP32
+// and BF16x3 were no faster, so they keep the three-ordering modes (and their code).
+__host__ __device__ constexpr bool pair_modes(int fk) { return fk != 1 && fk != 2; }
+
+// The mode of an item (wave-uniform) in the forms with pair_modes: f is called with
+// std::integral_constant<int, MODE>.  An orbit of three equal blocks accumulates one ordering
+// whatever the tuple; a degenerate tuple on an orbit with two equal blocks accumulates two (the
+// third of its three distinct orderings is a transpose of one of them).  The other forms select
+// the modes by an explicit chain at the call site (tkind 2 / 3 first: M_IJJ / M_IIJ on any
+// orbit); passing that chain through this helper's lambda changed their code generation
+// (scratch 260 -> 288 B/lane for FP32) and cost 2-3 %.
+template <class F>
+__device__ __forceinline__ void dispatch_mode(int tkind, int mult, const ItemCtx &it, F &&f)
+{
+    const bool aac = it.blk[0] == it.blk[1];
+    if (tkind == 2) {
+        if (mult == 1) f(std::integral_constant<int, M_IJJ>{});
+        else if (mult == 6) f(std::integral_constant<int, M_AAA>{});
+        else if (aac) f(std::integral_constant<int, M_IJJ_AAC>{});
+        else f(std::integral_constant<int, M_IJJ_ACC>{});
+    } else if (tkind == 3) {
+        if (mult == 1) f(std::integral_constant<int, M_IIJ>{});
+        else if (mult == 6) f(std::integral_constant<int, M_AAA>{});
+        else if (aac) f(std::integral_constant<int, M_IIJ_AAC>{});
+        else f(std::integral_constant<int, M_IIJ_ACC>{});
+    } else {
+        if (mult == 1) f(std::integral_constant<int, M_FULL>{});
+        else if (mult == 6) f(std::integral_constant<int, M_AAA>{});
+        else if (aac) f(std::integral_constant<int, M_AAC>{});
+        else f(std::integral_constant<int, M_ACC>{});
     }
 }
 

@@ -468,12 +468,18 @@ __global__ void __launch_bounds__(NTHR, 1) triples_kernel(TriplesArgs g)
         int tkind, mult;
         ItemCtx it = make_item_ctx(g.tuples, g.blocks, t, q, tkind, mult);
         const bool swap_bc = ragged_block_to_b(g, it, mult);
-        if (tkind == 2) do_item<M_IJJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
-        else if (tkind == 3) do_item<M_IIJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
-        else if (mult == 1) do_item<M_FULL, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
-        else if (mult == 6) do_item<M_AAA, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
-        else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
-        else do_item<M_ACC, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        if constexpr (pair_modes(FK)) {
+            dispatch_mode(tkind, mult, it, [&](auto M) __attribute__((always_inline)) {
+                do_item<decltype(M)::value, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+            });
+        } else {
+            if (tkind == 2) do_item<M_IJJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+            else if (tkind == 3) do_item<M_IIJ, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+            else if (mult == 1) do_item<M_FULL, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+            else if (mult == 6) do_item<M_AAA, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+            else if (it.blk[0] == it.blk[1]) do_item<M_AAC, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+            else do_item<M_ACC, T, FK>(g, lds, red, it, item, tkind, mult, swap_bc, wave, lane, tid);
+        }
     }
 }
 

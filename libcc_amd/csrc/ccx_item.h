@@ -630,6 +630,19 @@ __device__ __forceinline__ f4 split_mac(const split_frag_t<FK> &a, const split_f
     }
 }
 
+// An empty asm that reads and writes the accumulators of the orderings MODE computes: every
+// MFMA into them must be issued before it, and it stays where it is among the side effects
+// (barriers) around it.
+template <int MODE, int N>
+__device__ __forceinline__ void pin_accumulators(f4 (&acc)[N][BPW])
+{
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+        if ((mode_mask(MODE) >> p) & 1)
+#pragma unroll
+            for (int bb = 0; bb < BPW; ++bb) asm volatile("" : "+v"(acc[p][bb]));
+}
+
 // Blocks of KB K: the BX slices of the phase (shared by all waves) are staged in LDS, double
 // buffered across blocks; the AX slices stream per operand slot through a two-deep register
 // ring (two blocks per loop iteration keep the ring index a compile-time constant).  A slot
@@ -834,6 +847,10 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
             // the staged block must have landed before the barrier publishes it; the AX prefetch
             // of the next step (the NPF most recent loads; loads retire in issue order) may stay
             // in flight across the barrier
+            // the block's MFMAs stay before the barrier: without this the compiler may sink
+            // them past it, after the next block's loads, and keep both blocks' operands live
+            pin_accumulators<MODE>(g0);
+            if constexpr (CF::TWO_ACC) pin_accumulators<MODE>(g1);
             __builtin_amdgcn_s_waitcnt(WAIT_DMA);
             __syncthreads();
         });

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from kernel_model import (PERM, MODE_SET, blocks, complete, item_mode, pair_index, perm_index, term,
+from kernel_model import (PERM, MODE_SET, blocks, complete, item_mode, pair_index, perm_index, ragged_to_b, term,
                           tuples, wsrc_comp)
 from adjoint_model import SIGMA, ZTERMS, inv
 
@@ -58,9 +58,9 @@ def model_adjoint(inp: dict, tb: int = 4) -> dict:
         wt = 1.0 if kind == 1 else 0.5
         wbar = np.zeros((nup, nup, nup))
         for (A, B, C, mult) in blocks(nb):
-            blk = (A, B, C)
+            blk = ragged_to_b(nu, tb, A, B, C, mult)
             acc = np.zeros((6, tb, tb, tb))
-            mode = item_mode(kind, mult, A, B)
+            mode = item_mode(kind, mult, blk[0], blk[1])
             for p in MODE_SET[mode]:
                 for phase in range(3):
                     s1, b1, s2, b2 = term(phase, p)

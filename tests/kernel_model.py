@@ -57,19 +57,32 @@ def tuples(no: int) -> list:
 
 # orderings each item mode accumulates (the kernel's mode_mask)
 MODE_SET = {"full": (0, 1, 2, 3, 4, 5), "aac": (0, 1, 3), "acc": (0, 2, 4), "aaa": (0,),
-            "ijj": (0, 2, 3), "iij": (0, 1, 4)}
+            "ijj": (0, 2, 3), "iij": (0, 1, 4),
+            "ijj_aac": (0, 3), "ijj_acc": (0, 2), "iij_aac": (0, 1), "iij_acc": (0, 4)}
 
 
-def item_mode(kind: int, mult: int, A: int, B: int) -> str:
-    if kind == 2:
-        return "ijj"
-    if kind == 3:
-        return "iij"
+def item_mode(kind: int, mult: int, A: int, B: int, pair: bool = True) -> str:
+    """The kernel's mode selection; pair: the forms that combine a degenerate tuple's symmetry
+    with equal blocks (pair_modes: FP64, BF16x4, FP8x4)."""
+    if kind in (2, 3):
+        base = "ijj" if kind == 2 else "iij"
+        if mult == 1 or not pair:
+            return base
+        if mult == 6:
+            return "aaa"
+        return base + ("_aac" if A == B else "_acc")
     if mult == 1:
         return "full"
     if mult == 6:
         return "aaa"
     return "aac" if A == B else "acc"
+
+
+def ragged_to_b(nu: int, tb: int, A: int, B: int, C: int, mult: int) -> tuple:
+    """The kernel's ragged_block_to_b: a ragged last block of three distinct ones takes role B."""
+    if mult == 1 and nu - C * tb < tb:
+        return A, C, B
+    return A, B, C
 
 
 def complete(acc: np.ndarray, mode: str) -> np.ndarray:
@@ -78,6 +91,18 @@ def complete(acc: np.ndarray, mode: str) -> np.ndarray:
     if mode == "ijj":
         w[1], w[4], w[5] = w[0], w[2], w[3]
     elif mode == "iij":
+        w[2], w[3], w[5] = w[0], w[1], w[4]
+    elif mode == "ijj_aac":             # w2 = w0(b,a,c); then w1 = w0, w4 = w2, w5 = w3
+        w[2] = np.swapaxes(acc[0], 0, 1)
+        w[1], w[4], w[5] = w[0], w[2], w[3]
+    elif mode == "ijj_acc":             # w3 = w2(a,c,b); then w1 = w0, w4 = w2, w5 = w3
+        w[3] = np.swapaxes(acc[2], 1, 2)
+        w[1], w[4], w[5] = w[0], w[2], w[3]
+    elif mode == "iij_aac":             # w4 = w1(b,a,c); then w2 = w0, w3 = w1, w5 = w4
+        w[4] = np.swapaxes(acc[1], 0, 1)
+        w[2], w[3], w[5] = w[0], w[1], w[4]
+    elif mode == "iij_acc":             # w1 = w0(a,c,b); then w2 = w0, w3 = w1, w5 = w4
+        w[1] = np.swapaxes(acc[0], 1, 2)
         w[2], w[3], w[5] = w[0], w[1], w[4]
     elif mode == "aac":                 # w2(a,b,c)=w0(b,a,c), w4=w1(b,a,c), w5=w3(b,a,c)
         w[2], w[4], w[5] = (np.swapaxes(acc[q], 0, 1) for q in (0, 1, 3))
@@ -101,7 +126,7 @@ def blocks(nb: int) -> list:
     return out
 
 
-def model(inp: dict, tb: int = 4) -> dict:
+def model(inp: dict, tb: int = 4, pair: bool = True) -> dict:
     no, nu = inp["no"], inp["nu"]
     nup = -(-nu // tb) * tb
     nb = nup // tb
@@ -122,9 +147,10 @@ def model(inp: dict, tb: int = 4) -> dict:
     for (ti, tj, tk, kind) in tuples(no):
         tp = (ti, tj, tk)
         for (A, B, C, mult) in blocks(nb):
+            A, B, C = ragged_to_b(nu, tb, A, B, C, mult)
             sa, sb, sc = (slice(X * tb, X * tb + tb) for X in (A, B, C))
             acc = np.zeros((6, tb, tb, tb))
-            mode = item_mode(kind, mult, A, B)
+            mode = item_mode(kind, mult, A, B, pair)
             for p in MODE_SET[mode]:
                 for phase in range(3):
                     s1, b1, s2, b2 = term(phase, p)

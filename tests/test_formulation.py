@@ -10,10 +10,14 @@ from libcc_amd import synth
 from oracle import dense
 
 
+@pytest.mark.parametrize("pair", [True, False], ids=["paired-modes", "three-ordering-modes"])
 @pytest.mark.parametrize("no,nu,seed,tb", [(3, 5, 7, 4), (4, 6, 11, 4), (3, 9, 2, 4), (5, 7, 3, 2), (2, 6, 8, 2)])
-def test_kernel_model_matches_dense(no, nu, seed, tb):
+def test_kernel_model_matches_dense(no, nu, seed, tb, pair):
+    """Ragged blocks (nu not a multiple of tb) take role B; with pair, a degenerate tuple on an
+    orbit with equal blocks accumulates two orderings (one on A=B=C) and the kernel's
+    completion fills the rest from the cube's transposes and the tuple's pointwise equalities."""
     inp = synth.make_inputs(no, nu, seed)
-    m = km.model(inp, tb)
+    m = km.model(inp, tb, pair)
     r = dense.energies(inp)
     assert abs(m["etd"] - r["etd"]) <= 1e-14 * abs(r["etd"])
     assert abs(m["ets"] - r["ets"]) <= 1e-12 * abs(r["ets"]) + 1e-20

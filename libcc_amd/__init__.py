@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcc_mi355x.so")
@@ -35,6 +36,14 @@ def lib() -> ctypes.CDLL:
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `make -C libcc_amd` (no CPU fallback exists)")
+    # PyTorch bundles its own HIP runtime under the same soname.  Loaded first, it is the one
+    # runtime of the process and the library binds to it; loaded after the library has
+    # initialised the system runtime, torch finds no GPU.  So torch, where installed, goes first.
+    if "torch" not in sys.modules:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     L = ctypes.CDLL(LIB_PATH)
     dp, i64, vp = ctypes.POINTER(ctypes.c_double), ctypes.c_int64, ctypes.c_void_p
     ctxp = ctypes.c_void_p

@@ -728,12 +728,14 @@ __device__ __forceinline__ void run_phase_split(const TriplesArgs &g, const Item
             }
         }
     };
-    // A ring step is SPS operand slots and BS of the wave's batches: one slot of both batches for
-    // bf16, one slot and one batch at a time for the 32-byte fp8 fragments.  Two bf16 slots per
-    // step (a longer prefetch distance) measured 4 % (BF16x3) and 12 % (BF16x4) slower at no=24,
-    // nu=216: the larger ring costs registers, and the sweep is not waiting on its loads.
+    // A ring step is SPS operand slots and BS of the wave's batches: one slot of both batches.
+    // Two bf16 slots per step (a longer prefetch distance) measured 4 % (BF16x3) and 12 %
+    // (BF16x4) slower at no=24, nu=216: the larger ring costs registers, and the sweep is not
+    // waiting on its loads.  The 32-byte fp8 fragments took one batch per step while the
+    // compiler spilled the sweep; with the MFMAs pinned to their block (pin_accumulators) both
+    // batches fit: 879 ms against 1271 ms.
     typedef split_frag_t<FK> frag_t;
-    constexpr int BS = CF::FMT == 0 ? BPW : 1, NH = BPW / BS;
+    constexpr int BS = BPW, NH = BPW / BS;
     constexpr int SPS = 1;
     constexpr int NG = (NSL + SPS - 1) / SPS;          // slot groups per K block
     frag_t ab[2][SPS][BS][NS];

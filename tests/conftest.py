@@ -33,3 +33,19 @@ def golden():
 GOLDEN_CASES = [(3, 5, 7), (4, 6, 11), (5, 7, 3), (2, 9, 5), (6, 4, 21),
                 (4, 36, 1001), (4, 87, 1002), (8, 120, 1003)]
 SMALL_CASES = GOLDEN_CASES[:5]
+
+
+@pytest.fixture(scope="session")
+def progress(request):
+    """say(msg): a progress line on the terminal even under pytest -q with output captured, so a
+    test that runs for minutes is never silent for long."""
+    capman = request.config.pluginmanager.getplugin("capturemanager")
+
+    def say(msg: str) -> None:
+        if capman is None:
+            print(msg, flush=True)
+            return
+        with capman.global_and_fixture_disabled():
+            print(msg, flush=True)
+
+    return say

@@ -27,10 +27,10 @@ KEYS = ("t1", "t2", "vm", "voe", "vvvo")
 
 
 @pytest.fixture(scope="module")
-def c5():
+def c5(progress):
     import torch
 
-    print(f"generating no={NO} nu={NU} seed={SEED} on the device", flush=True)
+    progress(f"generating no={NO} nu={NU} seed={SEED} on the device")
     inp = synth.make_inputs_torch(NO, NU, SEED, "cuda:0")
     ctx = TriplesContext(NO, NU)
     yield torch, inp, ctx
@@ -40,21 +40,21 @@ def c5():
 
 
 @pytest.mark.timeout(900)
-def test_c5_forward_scaling_and_split(c5):
+def test_c5_forward_scaling_and_split(c5, progress):
     torch, inp, ctx = c5
     ctx.load(inp)
-    print("c5 fp64", flush=True)
+    progress("c5 fp64")
     r64 = ctx.energy("fp64")
-    print(f"c5 fp64 done: E(T) = {r64['e']:.10e}, kernel {ctx.timing()[0]:.0f} ms", flush=True)
+    progress(f"c5 fp64 done: E(T) = {r64['e']:.10e}, kernel {ctx.timing()[0]:.0f} ms")
     rs = ctx.energy(SPLIT)
     de = abs(rs["e"] - r64["e"])
-    print(f"c5 {SPLIT} done: |dE| = {de:.3e} Eh ({de / abs(r64['e']):.2e} relative), kernel "
-          f"{ctx.timing()[0]:.0f} ms", flush=True)
+    progress(f"c5 {SPLIT} done: |dE| = {de:.3e} Eh ({de / abs(r64['e']):.2e} relative), kernel "
+             f"{ctx.timing()[0]:.0f} ms")
     assert de <= 1e-9 * abs(r64["e"])
     sc = dict(inp, vvvo=inp["vvvo"] * 2.0, vm=inp["vm"] * 2.0)
     ctx.load(sc)
     for form, ref in (("fp64", r64), (SPLIT, rs)):
-        print(f"c5 doubled {form}", flush=True)
+        progress(f"c5 doubled {form}")
         r2 = ctx.energy(form)
         assert r2["etd"] == 4.0 * ref["etd"], form
         assert r2["ets"] == 2.0 * ref["ets"], form
@@ -64,22 +64,22 @@ def test_c5_forward_scaling_and_split(c5):
 
 
 @pytest.mark.timeout(900)
-def test_c5_split_adjoint(c5):
+def test_c5_split_adjoint(c5, progress):
     torch, inp, ctx = c5
     ctx.load(inp)
     r64 = ctx.energy("fp64")
-    print("c5 adjoint fp64", flush=True)
+    progress("c5 adjoint fp64")
     g64 = ctx.gradient(device=True)
-    print(f"c5 adjoint fp64 done: {ctx.timing()[0]:.0f} ms", flush=True)
+    progress(f"c5 adjoint fp64 done: {ctx.timing()[0]:.0f} ms")
     assert abs(g64["e"] - r64["e"]) <= 1e-12 * abs(r64["e"])
     v1 = torch.as_tensor(np.ascontiguousarray(r64["v1"].T), device="cuda:0")
     assert float((g64["t1"] - 2.0 * v1).abs().max()) <= 1e-12 * float(v1.abs().max())
-    print(f"c5 adjoint {SPLIT}", flush=True)
+    progress(f"c5 adjoint {SPLIT}")
     g = ctx.gradient(device=True, form=SPLIT)
-    print(f"c5 adjoint {SPLIT} done: {ctx.timing()[0]:.0f} ms", flush=True)
+    progress(f"c5 adjoint {SPLIT} done: {ctx.timing()[0]:.0f} ms")
     for k in KEYS:
         err = float(torch.linalg.norm(g[k] - g64[k]) / torch.linalg.norm(g64[k]))
-        print(f"  d{k}: {err:.2e}")
+        progress(f"  d{k}: {err:.2e}")
         assert err <= 1e-6, (k, err)
     del g, g64
     torch.cuda.empty_cache()
